@@ -1,0 +1,192 @@
+/*
+ * ocfd_hip.h -- C ABI of the MI355X (gfx950) right-hand-side + RK3 path of the
+ * OpenCFD-SC compressible Navier-Stokes solver.
+ *
+ * The reference solver is Fortran 90 whose hot-path routines take no dummy
+ * arguments and work on module globals (OCFD_parameters.f90:70-108).  This
+ * library keeps the same data in device memory and exposes one entry point per
+ * reference routine, so the Fortran driver can replace the routine bodies with
+ * a single call each (iso_c_binding), plus whole-step entry points for callers
+ * that keep the state device-resident.
+ *
+ * Conventions
+ *  - Every entry point returns int: 0 = OK, < 0 = configuration / HIP error
+ *    (message available from ocfd_last_error), > 0 = a numerical event
+ *    (e.g. the number of cells with T <= 0 found by ocfd_comput_duvwt).
+ *  - Host arrays use the reference's Fortran layout: halo'd fields are
+ *    X(1-LAP:nx+LAP, 1-LAP:ny+LAP, 1-LAP:nz+LAP), conserved-variable arrays are
+ *    f(nx,ny,nz,5); column-major, i fastest.  The library never keeps or frees
+ *    a host pointer; it copies in/out synchronously.
+ *  - One handle = one block on one GPU; calls on one handle are serialised on
+ *    the handle's HIP stream and are not thread-safe.
+ */
+#ifndef OCFD_HIP_H
+#define OCFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scheme / splitting / BC codes: same values as OCFD_constants
+ * (OCFD_parameters.f90:23-31). */
+enum {
+    OCFD_SCHEME_WENO5 = 1,
+    OCFD_SCHEME_WENO7 = 2,
+    OCFD_SCHEME_OMP6 = 3,
+    OCFD_SCHEME_UD7L = 4,
+    OCFD_SCHEME_CD6 = 5,
+    OCFD_SCHEME_CD8 = 6,
+    OCFD_SCHEME_HYBRID = 10
+};
+enum { OCFD_SPLIT_SW = 1, OCFD_SPLIT_LLF = 2 };
+enum { OCFD_BC_NONE = 0, OCFD_BC_BLUNT2D = 1, OCFD_BC_USER_DEF = 99 };
+/* Arithmetic of the right-hand side. */
+enum { OCFD_PREC_FP64 = 0, OCFD_PREC_MIXED = 1 };
+/* How grid metrics are held on the device. */
+enum { OCFD_METRIC_FIELD = 0, OCFD_METRIC_UNIFORM = 1 };
+
+/*
+ * Run parameters: the namelist &control_opencfd after read_parameter
+ * (OCFD_readpara.f90:3-200), for a single block (npx0=npy0=npz0=1).
+ * Derived quantities (hx, Cv, Cp, stencil widths, UD7L coefficients,
+ * boundary-scheme flags) are computed by the library exactly as
+ * set_parameters (OCFD_readpara.f90:261-341) does.
+ */
+typedef struct ocfd_config {
+    int32_t nx, ny, nz;            /* nx_global, ny_global, nz_global          */
+    int32_t lap;                   /* halo width LAP; must be 4                 */
+    int32_t iperiodic[3];          /* Iperiodic_X/Y/Z (1 = periodic)           */
+    int32_t scheme_invis;          /* OCFD_SCHEME_WENO5/WENO7/OMP6/UD7L/HYBRID  */
+    int32_t scheme_vis;            /* OCFD_SCHEME_CD6 / OCFD_SCHEME_CD8         */
+    int32_t flux_splitting;        /* OCFD_SPLIT_SW / OCFD_SPLIT_LLF            */
+    int32_t if_scheme_character;   /* IF_Scheme_Character                       */
+    int32_t if_viscous;            /* IF_Viscous                                */
+    int32_t ibc;                   /* BoundaryCondition code                    */
+    int32_t if_mass_force;         /* IF_Mass_Force                             */
+    int32_t scheme_boundary[6];    /* Scheme_boundary(1:6)                      */
+    int32_t ghost_cell[6];         /* Ghost_Cell(1:6)                           */
+    int32_t nfiltering;            /* NFiltering (<= 10)                        */
+    int32_t istep_show;            /* Istep_Show                                */
+    int32_t precision;             /* OCFD_PREC_*                               */
+    int32_t metric_mode;           /* OCFD_METRIC_*                             */
+    int32_t reserved_i[8];
+    double ma, re, gamma, pr, ref_amu_t0, dt;
+    double ud7l_diss;              /* UD7L_Diss                                 */
+    double mass_force[3];
+    double bc_para[100];           /* BC_Para(1:100)                            */
+    double hybrid_para[100];       /* Hybrid_para(1:100)                        */
+    double filter_para[10][100];   /* Filter_Para(1:100, m) at [m-1][0..99]     */
+    double reserved_d[8];
+} ocfd_config;
+
+typedef struct ocfd_handle ocfd_handle;
+
+/* ---- lifetime --------------------------------------------------------- */
+/* Replaces allocate_flow_data / allocate_inviscous_data /
+ * allocate_vicous_data (OCFD_NS_Solver.f90:93-151, OCFD_inviscous.f90:15-40,
+ * OCFD_viscous.f90:11-20): allocates every device array of one block. */
+int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out);
+/* Replaces the deallocate_* routines. */
+int ocfd_destroy(ocfd_handle *h);
+/* Text of the last error on this thread (never NULL). */
+const char *ocfd_last_error(void);
+/* Block until all work queued on the handle's stream has finished. */
+int ocfd_synchronize(ocfd_handle *h);
+
+/* ---- grid / metrics (init_mesh, OCFD_mesh.f90:1-58) -------------------- */
+/* Interior coordinates Axx,Ayy,Azz (nx*ny*nz each, Fortran order) plus the
+ * periodic spans Periodic_I/J/KSpan (9 doubles): the device fills the halos,
+ * applies updata_Axyz_periodic (OCFD_mesh.f90:109-185), computes
+ * comput_Jacobian3d (OCFD_mesh.f90:61-107) with OCFD_dx0/dy0/dz0, the metric
+ * halos, Jac_Ghost_boundary (OCFD_GhostCell.f90:1-160, 290-391) and the
+ * normalised metrics Akx1=Akx/Ajac ... (OCFD_mesh.f90:41-55). */
+int ocfd_set_grid(ocfd_handle *h, const double *x, const double *y, const double *z,
+                  const double spans[9]);
+/* Upload metrics computed elsewhere: 19 halo'd arrays in the order
+ * Akx,Aky,Akz,Aix,Aiy,Aiz,Asx,Asy,Asz,Ajac,Akx1,Aky1,Akz1,Aix1,Aiy1,Aiz1,
+ * Asx1,Asy1,Asz1 (the state of flow_data after init_mesh). */
+int ocfd_upload_metrics(ocfd_handle *h, const double *const metrics[19]);
+/* OCFD_METRIC_UNIFORM only: the ten constant values Akx,Aky,Akz,Aix,Aiy,Aiz,
+ * Asx,Asy,Asz,Ajac (the reference's GRID_AND_JACOBIAN3D input holding one
+ * value per field); normalised metrics are formed as Akx/Ajac ... */
+int ocfd_set_uniform_metrics(ocfd_handle *h, const double m[10]);
+int ocfd_download_metrics(ocfd_handle *h, double *const metrics[19]);
+
+/* ---- state ------------------------------------------------------------ */
+/* Upload primitives (halo'd d,u,v,w,T; the halo content is copied too) and
+ * build f from them as init3d does (OCFD_init.f90:10-20); Istep, tt set the
+ * step counter and time. */
+int ocfd_upload_primitives(ocfd_handle *h, const double *d, const double *u,
+                           const double *v, const double *w, const double *T,
+                           int32_t istep, double tt);
+/* Upload / download the conserved variables f(nx,ny,nz,5). */
+int ocfd_upload_f(ocfd_handle *h, const double *f);
+int ocfd_download_f(ocfd_handle *h, double *f);
+int ocfd_download_fn(ocfd_handle *h, double *fn);
+int ocfd_download_du(ocfd_handle *h, double *du);
+int ocfd_upload_du(ocfd_handle *h, const double *du);
+/* Download halo'd primitives (any pointer may be NULL). */
+int ocfd_download_primitives(ocfd_handle *h, double *d, double *u, double *v,
+                             double *w, double *T);
+int ocfd_download_rhybrid(ocfd_handle *h, double *rhybrid);
+int ocfd_get_time(ocfd_handle *h, int32_t *istep, double *tt);
+
+/* ---- one reference routine each --------------------------------------- */
+/* exchange_boundary_xyz(d,u,v,w,T) (OCFD_NS_Solver.f90:35-39,
+ * OCFD_mpi.f90:183-233): on one block the periodic halo copy. */
+int ocfd_exchange_primitives(ocfd_handle *h);
+/* comput_Rhybrid (OCFD_SetHybrid.f90:222-357).  counts (may be NULL) gets the
+ * number of faces using scheme 1/2/3 and the total, as printed at
+ * OCFD_SetHybrid.f90:320-342. */
+int ocfd_comput_rhybrid(ocfd_handle *h, int64_t counts[4]);
+/* du_inviscous (OCFD_inviscous.f90:53-203) */
+int ocfd_du_inviscous(ocfd_handle *h);
+/* du_inviscous_Character (OCFD_inviscous_character.f90:3-159) */
+int ocfd_du_inviscous_character(ocfd_handle *h);
+/* du_viscous (OCFD_viscous.f90:29-131) */
+int ocfd_du_viscous(ocfd_handle *h);
+/* OCFD_time_adv_RK3(KRK) (OCFD_Time_Adv.f90:4-36). */
+int ocfd_time_adv_rk3(ocfd_handle *h, int32_t krk);
+/* fn = f (OCFD_NS_Solver.f90:23-31) */
+int ocfd_save_fn(ocfd_handle *h);
+/* comput_duvwT (OCFD_Time_Adv.f90:40-63): returns the number of cells with
+ * T <= 0 (> 0 is a numerical event; those cells are repaired as
+ * handle_NegativeT does, OCFD_handle_NegativeT.f90:1-17). */
+int ocfd_comput_duvwt(ocfd_handle *h);
+/* OCFD_bc (OCFD_BC.f90:2-99) incl. Flow_Ghost_boundary. */
+int ocfd_bc(ocfd_handle *h);
+/* filtering(f) (OCFD_filtering.f90:4-70) at the handle's current Istep. */
+int ocfd_filtering(ocfd_handle *h);
+/* Advance Istep by one and tt by dt (OCFD_NS_Solver.f90:64-65). */
+int ocfd_advance_clock(ocfd_handle *h);
+
+/* ---- whole steps ------------------------------------------------------ */
+/* nsteps iterations of the time loop body (OCFD_NS_Solver.f90:23-67): fn=f,
+ * three RK stages, Istep/tt update and filtering.  State stays on device. */
+int ocfd_step(ocfd_handle *h, int32_t nsteps);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* show_flow_msg sums (OCFD_Time_Adv.f90:65-90): out[0..2] = E, K, S
+ * (averaged over the global cell count). */
+int ocfd_show_flow_msg(ocfd_handle *h, double out[3]);
+/* ana_box statistics (OCFD_ANA.f90:178-311): out[0..19] in the column order
+ * of statistics.dat: tt, Re_lamda, Re_lamdax, Amt, vv_rms, u_rms, v_rms,
+ * w_rms, u_skewness, u_flatness, ux_skewness, ux_flatness, E_kinetic,
+ * Enstrophy, d_rms, T_aver, Amu_aver, d_aver, Ux_rms, Alamdax. */
+int ocfd_ana_box(ocfd_handle *h, double out[20]);
+
+/* ---- instrumentation -------------------------------------------------- */
+/* Enable per-kernel HIP-event timing on the handle's stream. */
+int ocfd_profile_enable(ocfd_handle *h, int32_t on);
+/* Per-kernel accumulated time (ms) and launch count; names are the kernel
+ * families ("inviscid_x", ...).  Returns the number of entries written. */
+int ocfd_profile_read(ocfd_handle *h, int32_t max_entries, char names[][32],
+                      double *ms, int64_t *launches);
+int ocfd_profile_reset(ocfd_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCFD_HIP_H */
