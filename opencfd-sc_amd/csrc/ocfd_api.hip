@@ -1,0 +1,641 @@
+// ocfd_api.hip -- implementation of the C ABI declared in include/ocfd_hip.h.
+//
+// One handle owns one block's device arrays (the reference's flow_data module,
+// OCFD_parameters.f90:99-108, plus the work arrays of the inviscid and viscous
+// modules) on one GPU and one HIP stream.  Host arrays are only copied in/out.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ocfd_hip.h"
+#include "ocfd_kernels.h"
+
+using namespace ocfd;
+
+namespace {
+thread_local std::string g_err = "";
+
+int fail(const std::string &msg) {
+  g_err = msg;
+  return -1;
+}
+
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+};
+}  // namespace
+
+struct ocfd_handle {
+  ocfd_config cfg;
+  int device = 0;
+  hipStream_t st = nullptr;
+  Block B;
+  double *met[M_N] = {};
+  double *fbuf = nullptr;
+  double *work = nullptr;      // reduction partials
+  double *wout = nullptr;      // reduction results (32 doubles)
+  int *neg = nullptr;          // [0] stage count, [1..64] cells, [65] total, [66] max per stage
+  unsigned long long *counts = nullptr;
+  int ibc = 0;
+  double tw = 0.0;
+  int ghost[6] = {};
+  int istep = 0;
+  double tt = 0.0;
+  bool prof = false;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> acc;
+  size_t halo_bytes() const { return (size_t)B.g.nh * sizeof(double); }
+  size_t f_bytes() const { return (size_t)5 * B.g.ni * sizeof(double); }
+};
+
+namespace {
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+hipEvent_t ev_get(ocfd_handle *h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// run one kernel launch, timing it on the handle's stream when profiling
+template <typename F>
+int launch(ocfd_handle *h, const char *name, F &&fn) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (h->prof) {
+    a = ev_get(h);
+    b = ev_get(h);
+    (void)hipEventRecord(a, h->st);
+  }
+  hipError_t e = fn();
+  if (h->prof) {
+    (void)hipEventRecord(b, h->st);
+    h->pending.push_back({name, a, b});
+  }
+  if (e != hipSuccess) return fail(std::string(name) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+int flush_prof(ocfd_handle *h) {
+  if (h->pending.empty()) return 0;
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (auto &r : h->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto &slot = h->acc[r.name];
+    slot.first += ms;
+    slot.second += 1;
+    h->pool.push_back(r.a);
+    h->pool.push_back(r.b);
+  }
+  h->pending.clear();
+  return 0;
+}
+
+bool valid_scheme(int s) { return s == 1 || s == 2 || s == 3 || s == 4 || s == 10; }
+
+int set_params(ocfd_handle *h) {
+  const ocfd_config &c = h->cfg;
+  if (c.lap != LAP) return fail("LAP must be 4");
+  if (c.nx < 8 || c.ny < 8 || c.nz < 8) return fail("each block extent must be >= 8");
+  if (!valid_scheme(c.scheme_invis)) return fail("Scheme_Invis not supported");
+  if (c.scheme_vis != SCH_CD6 && c.scheme_vis != SCH_CD8) return fail("Scheme_Vis must be CD6 or CD8");
+  if (c.flux_splitting != 1 && c.flux_splitting != 2) return fail("Flux_Splitting must be 1 or 2");
+  if (c.ibc != 0 && c.ibc != 1 && c.ibc != 99) return fail("BoundaryCondition not supported");
+  if (c.nfiltering < 0 || c.nfiltering > 10) return fail("NFiltering must be <= 10");
+  if (c.precision != OCFD_PREC_FP64) return fail("only OCFD_PREC_FP64 is available in this build");
+  for (int q = 0; q < 6; q++)
+    if (c.scheme_boundary[q] < -1 || c.scheme_boundary[q] > 1) return fail("Scheme_boundary must be -1, 0 or 1");
+  Block &B = h->B;
+  Geom &g = B.g;
+  g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+  g.sx = c.nx + 2 * LAP; g.sy = c.ny + 2 * LAP; g.sz = c.nz + 2 * LAP;
+  g.nh = (int64_t)g.sx * g.sy * g.sz;
+  g.ni = (int64_t)g.nx * g.ny * g.nz;
+  // set_parameters (OCFD_readpara.f90:261-341)
+  SchemeParams &S = B.S;
+  S.scheme_invis = c.scheme_invis;
+  S.scheme_vis = c.scheme_vis;
+  S.flux_split = c.flux_splitting;
+  S.is_char = c.if_scheme_character;
+  switch (c.scheme_invis) {
+    case SCH_WENO5: S.ka1 = -2; S.kb1 = 2; S.ka2 = -1; S.kb2 = 3; break;
+    case SCH_WENO7: S.ka1 = -3; S.kb1 = 3; S.ka2 = -2; S.kb2 = 4; break;
+    default: S.ka1 = -3; S.kb1 = 4; S.ka2 = -3; S.kb2 = 4; break;
+  }
+  for (int a = 0; a < 3; a++) {
+    S.iperiodic[a] = c.iperiodic[a];
+    S.bidx[a][0] = S.bidx[a][1] = c.iperiodic[a] == 0 ? 1 : 0;
+  }
+  for (int q = 0; q < 6; q++) S.sb[q] = c.scheme_boundary[q];
+  const double ud7[8] = {-3.0, 25.0, -101.0, 319.0, 214.0, -38.0, 4.0, 0.0};
+  const double cd8[8] = {-3.0, 29.0, -139.0, 533.0, 533.0, -139.0, 29.0, -3.0};
+  for (int q = 0; q < 8; q++) S.ud7l[q] = c.ud7l_diss * ud7[q] / 420.0 + (1.0 - c.ud7l_diss) * cd8[q] / 840.0;
+  S.hyb2 = c.hybrid_para[1];
+  S.hyb3 = c.hybrid_para[2];
+  Phys &P = B.P;
+  P.gamma = c.gamma; P.ma = c.ma; P.re = c.re; P.pr = c.pr; P.dt = c.dt; P.ref_amu_t0 = c.ref_amu_t0;
+  P.h[0] = 1.0 / ((double)c.nx - 1.0);
+  P.h[1] = 1.0 / ((double)c.ny - 1.0);
+  P.h[2] = 1.0 / ((double)c.nz - 1.0);
+  for (int a = 0; a < 3; a++) P.hinv[a] = 1.0 / P.h[a];
+  P.cv = 1.0 / (c.gamma * (c.gamma - 1.0) * c.ma * c.ma);
+  P.cp = P.cv * c.gamma;
+  P.if_mass_force = c.if_mass_force;
+  for (int a = 0; a < 3; a++) P.mass_force[a] = c.mass_force[a];
+  h->ibc = c.ibc;
+  h->tw = c.bc_para[0];
+  for (int q = 0; q < 6; q++) h->ghost[q] = c.ghost_cell[q];
+  return 0;
+}
+
+__global__ void k_fill(double *a, int64_t n, double v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+hipError_t fill(double *a, int64_t n, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_int(int *a, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0;
+}
+
+// forced-WENO patches of the sensor (OCFD_SetHybrid.f90:293-315), with the j
+// range read from Hybrid_para(7:8) as the reference's comments intend
+__global__ void k_patch(const Geom g, double *r, int ib, int ie, int jb, int je, int kb, int ke) {
+  const int64_t ni = (int64_t)(ie - ib + 1) * (je - jb + 1) * (ke - kb + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ni) return;
+  const int w = ie - ib + 1, hgt = je - jb + 1;
+  const int i = ib + (int)(t % w), j = jb + (int)((t / w) % hgt), k = kb + (int)(t / ((int64_t)w * hgt));
+  const int64_t q = g.hi(i, j, k);
+  r[q] = r[q] + 100.0;
+}
+
+int alloc(double **p, size_t bytes) {
+  if (hipMalloc((void **)p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return fail("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return 0;
+}
+
+double *prims_of(ocfd_handle *h, int q) {
+  double *p[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
+  return p[q];
+}
+
+int inviscid(ocfd_handle *h, bool ch) {
+  Block B = h->B;
+  B.S.is_char = ch ? 1 : 0;
+  static const char *nm[2][3] = {{"inviscid_x", "inviscid_y", "inviscid_z"},
+                                 {"inviscid_char_x", "inviscid_char_y", "inviscid_char_z"}};
+  const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
+  for (int a = 0; a < 3; a++)
+    if (launch(h, nm[ch][a], [&] { return launch_inviscid(B, a, modes[a], SweepTile{0, 0}, h->st); })) return -1;
+  return 0;
+}
+
+int exchange_prims(ocfd_handle *h) {
+  double *f[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
+  return launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, f, 5, h->st); });
+}
+
+int sensor(ocfd_handle *h) {
+  const ocfd_config &c = h->cfg;
+  const int kind = (int)std::lround(c.hybrid_para[0]);
+  if (kind == 1) {
+    if (launch(h, "sensor", [&] { return launch_sensor(h->B, h->st); })) return -1;
+  } else {
+    if (launch(h, "sensor", [&] { return launch_sensor2(h->B, h->work, h->st); })) return -1;
+  }
+  const int mpatch = (int)std::lround(c.hybrid_para[3]);
+  for (int m = 1; m <= mpatch; m++) {
+    const double *hp = c.hybrid_para;
+    int ib = (int)std::lround(hp[4 + (m - 1) * 6]), ie = (int)std::lround(hp[5 + (m - 1) * 6]);
+    int jb = (int)std::lround(hp[6 + (m - 1) * 6]), je = (int)std::lround(hp[7 + (m - 1) * 6]);
+    int kb = (int)std::lround(hp[8 + (m - 1) * 6]), ke = (int)std::lround(hp[9 + (m - 1) * 6]);
+    ib = ib < 1 ? 1 : ib; jb = jb < 1 ? 1 : jb; kb = kb < 1 ? 1 : kb;
+    ie = ie > c.nx ? c.nx : ie; je = je > c.ny ? c.ny : je; ke = ke > c.nz ? c.nz : ke;
+    if (ie < ib || je < jb || ke < kb) continue;
+    const int64_t n = (int64_t)(ie - ib + 1) * (je - jb + 1) * (ke - kb + 1);
+    hipLaunchKernelGGL(k_patch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->st, h->B.g, h->B.rhyb, ib, ie,
+                       jb, je, kb, ke);
+  }
+  double *r[1] = {h->B.rhyb};
+  return launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, r, 1, h->st); });
+}
+
+int filtering(ocfd_handle *h) {
+  const ocfd_config &c = h->cfg;
+  for (int m = 0; m < c.nfiltering; m++) {
+    const double *fp = c.filter_para[m];
+    const int per = (int)std::lround(fp[0]);
+    if (per == 0 || h->istep % per != 0) continue;
+    int rng[6];
+    for (int q = 0; q < 6; q++) rng[q] = (int)std::lround(fp[4 + q]);
+    if (rng[0] < 1) rng[0] = 1;
+    if (rng[1] > c.nx) rng[1] = c.nx;
+    if (rng[2] < 1) rng[2] = 1;
+    if (rng[3] > c.ny) rng[3] = c.ny;
+    if (rng[4] < 1) rng[4] = 1;
+    if (rng[5] > c.nz) rng[5] = c.nz;
+    const int scheme = (int)std::lround(fp[10]);
+    if (scheme != 1 && scheme != 2) continue;
+    for (int a = 0; a < 3; a++) {
+      if ((int)std::lround(fp[1 + a]) != 1) continue;
+      if (launch(h, "filter", [&] {
+            return launch_filter(h->B, a, scheme, fp[11], fp[12], rng, h->fbuf, nullptr, h->st);
+          }))
+        return -1;
+      std::swap(h->B.f, h->fbuf);
+    }
+  }
+  return 0;
+}
+
+int read_neg(ocfd_handle *h, int out[3]) {
+  HIPCHK(hipMemcpyAsync(out, h->neg, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(out + 1, h->neg + 65, 2 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *ocfd_last_error(void) { return g_err.c_str(); }
+
+int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
+  if (!cfg || !out) return fail("null argument");
+  ocfd_handle *h = new ocfd_handle();
+  h->cfg = *cfg;
+  if (set_params(h)) { delete h; return -1; }
+  if (cfg->metric_mode != OCFD_METRIC_FIELD && cfg->metric_mode != OCFD_METRIC_UNIFORM) {
+    delete h;
+    return fail("metric_mode must be FIELD or UNIFORM");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { delete h; return fail("no HIP device"); }
+  if (device < 0 || device >= ndev) { delete h; return fail("device index out of range"); }
+  h->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&h->st) != hipSuccess) {
+    delete h;
+    return fail("cannot create HIP stream");
+  }
+  Block &B = h->B;
+  const size_t hb = h->halo_bytes(), fb = h->f_bytes();
+  int rc = 0;
+  rc |= alloc(&B.d, hb) | alloc(&B.u, hb) | alloc(&B.v, hb) | alloc(&B.w, hb) | alloc(&B.T, hb);
+  rc |= alloc(&B.f, fb) | alloc(&B.fn, fb) | alloc(&B.du, fb);
+  if (cfg->scheme_invis == SCH_HYBRID) rc |= alloc(&B.rhyb, hb);
+  if (cfg->if_viscous == 1) rc |= alloc(&B.ev, 12 * (size_t)B.g.ni * sizeof(double));
+  if (cfg->nfiltering > 0) rc |= alloc(&h->fbuf, fb);
+  B.M.uniform = cfg->metric_mode == OCFD_METRIC_UNIFORM;
+  if (!B.M.uniform)
+    for (int m = 0; m < M_N; m++) rc |= alloc(&h->met[m], hb);
+  const int64_t nblk = (B.g.ni + 255) / 256;
+  rc |= alloc(&h->work, (size_t)(nblk * 14 + 64) * sizeof(double));
+  rc |= alloc(&h->wout, 32 * sizeof(double));
+  if (hipMalloc((void **)&h->neg, 72 * sizeof(int)) != hipSuccess) rc = -1;
+  if (hipMalloc((void **)&h->counts, 4 * sizeof(unsigned long long)) != hipSuccess) rc = -1;
+  if (rc) {
+    std::string msg = g_err;
+    ocfd_destroy(h);
+    return fail(msg.empty() ? "allocation failed" : msg);
+  }
+  for (int m = 0; m < M_N; m++) {
+    B.M.p[m] = h->met[m];
+    B.M.u[m] = 1.0;
+  }
+  // allocate_flow_data initial values (OCFD_NS_Solver.f90:116-118, 140-147)
+  double *hal[5] = {B.d, B.u, B.v, B.w, B.T};
+  for (int q = 0; q < 5; q++) (void)fill(hal[q], B.g.nh, 1.0, h->st);
+  if (!B.M.uniform)
+    for (int m = 0; m < M_N; m++) (void)fill(h->met[m], B.g.nh, 1.0, h->st);
+  if (B.rhyb) (void)fill(B.rhyb, B.g.nh, 0.0, h->st);
+  (void)fill(B.du, 5 * B.g.ni, 0.0, h->st);
+  (void)fill(B.f, 5 * B.g.ni, 0.0, h->st);
+  (void)fill(B.fn, 5 * B.g.ni, 0.0, h->st);
+  if (B.ev) (void)fill(B.ev, 12 * B.g.ni, 0.0, h->st);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  if (hipStreamSynchronize(h->st) != hipSuccess) {
+    ocfd_destroy(h);
+    return fail("initialisation failed");
+  }
+  *out = h;
+  return 0;
+}
+
+int ocfd_destroy(ocfd_handle *h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->device);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  Block &B = h->B;
+  double *ps[] = {B.d, B.u, B.v, B.w, B.T, B.f, B.fn, B.du, B.rhyb, B.ev, h->fbuf, h->work, h->wout};
+  for (double *p : ps)
+    if (p) (void)hipFree(p);
+  for (int m = 0; m < M_N; m++)
+    if (h->met[m]) (void)hipFree(h->met[m]);
+  if (h->neg) (void)hipFree(h->neg);
+  if (h->counts) (void)hipFree(h->counts);
+  for (auto &r : h->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return 0;
+}
+
+int ocfd_synchronize(ocfd_handle *h) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+// ---- grid / metrics --------------------------------------------------------
+int ocfd_set_grid(ocfd_handle *h, const double *x, const double *y, const double *z, const double spans[9]) {
+  if (h->B.M.uniform) return fail("ocfd_set_grid needs OCFD_METRIC_FIELD");
+  const Geom &g = h->B.g;
+  std::vector<double> stage((size_t)g.nh);
+  double *dc[3] = {};
+  const double *src[3] = {x, y, z};
+  int rc = 0;
+  for (int c = 0; c < 3 && !rc; c++) {
+    rc = alloc(&dc[c], h->halo_bytes());
+    if (rc) break;
+    std::fill(stage.begin(), stage.end(), 1.0);
+    for (int k = 1; k <= g.nz; k++)
+      for (int j = 1; j <= g.ny; j++)
+        std::memcpy(&stage[(size_t)g.hi(1, j, k)], &src[c][(size_t)g.ii(1, j, k)], (size_t)g.nx * sizeof(double));
+    if (hipMemcpy(dc[c], stage.data(), h->halo_bytes(), hipMemcpyHostToDevice) != hipSuccess) rc = fail("copy");
+  }
+  if (!rc) {
+    for (int m = 0; m < M_N; m++) (void)fill(h->met[m], g.nh, 1.0, h->st);
+    rc = launch(h, "metrics", [&] { return launch_metrics(h->B, dc, h->met, spans, h->ghost, nullptr, h->st); });
+    if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail("metrics kernels failed");
+  }
+  for (int c = 0; c < 3; c++)
+    if (dc[c]) (void)hipFree(dc[c]);
+  return rc;
+}
+
+int ocfd_upload_metrics(ocfd_handle *h, const double *const metrics[19]) {
+  if (h->B.M.uniform) return fail("ocfd_upload_metrics needs OCFD_METRIC_FIELD");
+  for (int m = 0; m < M_N; m++) HIPCHK(hipMemcpy(h->met[m], metrics[m], h->halo_bytes(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ocfd_set_uniform_metrics(ocfd_handle *h, const double m[10]) {
+  if (!h->B.M.uniform) return fail("ocfd_set_uniform_metrics needs OCFD_METRIC_UNIFORM");
+  for (int q = 0; q < 10; q++) h->B.M.u[q] = m[q];
+  for (int q = 0; q < 9; q++) h->B.M.u[M_AKX1 + q] = m[q] / m[M_AJAC];
+  return 0;
+}
+
+int ocfd_download_metrics(ocfd_handle *h, double *const metrics[19]) {
+  const Geom &g = h->B.g;
+  for (int m = 0; m < M_N; m++) {
+    if (!metrics[m]) continue;
+    if (h->B.M.uniform) {
+      for (int64_t q = 0; q < g.nh; q++) metrics[m][q] = h->B.M.u[m];
+    } else {
+      HIPCHK(hipMemcpy(metrics[m], h->met[m], h->halo_bytes(), hipMemcpyDeviceToHost));
+    }
+  }
+  return 0;
+}
+
+// ---- state -----------------------------------------------------------------
+int ocfd_upload_primitives(ocfd_handle *h, const double *d, const double *u, const double *v, const double *w,
+                           const double *T, int32_t istep, double tt) {
+  const double *src[5] = {d, u, v, w, T};
+  for (int q = 0; q < 5; q++) HIPCHK(hipMemcpy(prims_of(h, q), src[q], h->halo_bytes(), hipMemcpyHostToDevice));
+  h->istep = istep;
+  h->tt = tt;
+  if (launch(h, "init_f", [&] { return launch_init_f(h->B, h->st); })) return -1;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int ocfd_upload_f(ocfd_handle *h, const double *f) {
+  HIPCHK(hipMemcpy(h->B.f, f, h->f_bytes(), hipMemcpyHostToDevice));
+  return 0;
+}
+int ocfd_download_f(ocfd_handle *h, double *f) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(f, h->B.f, h->f_bytes(), hipMemcpyDeviceToHost));
+  return 0;
+}
+int ocfd_download_fn(ocfd_handle *h, double *f) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(f, h->B.fn, h->f_bytes(), hipMemcpyDeviceToHost));
+  return 0;
+}
+int ocfd_download_du(ocfd_handle *h, double *f) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(f, h->B.du, h->f_bytes(), hipMemcpyDeviceToHost));
+  return 0;
+}
+int ocfd_upload_du(ocfd_handle *h, const double *f) {
+  HIPCHK(hipMemcpy(h->B.du, f, h->f_bytes(), hipMemcpyHostToDevice));
+  return 0;
+}
+int ocfd_download_primitives(ocfd_handle *h, double *d, double *u, double *v, double *w, double *T) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  double *dst[5] = {d, u, v, w, T};
+  for (int q = 0; q < 5; q++)
+    if (dst[q]) HIPCHK(hipMemcpy(dst[q], prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+  return 0;
+}
+int ocfd_download_rhybrid(ocfd_handle *h, double *r) {
+  if (!h->B.rhyb) return fail("Rhybrid exists only for the HYBRID scheme");
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(r, h->B.rhyb, h->halo_bytes(), hipMemcpyDeviceToHost));
+  return 0;
+}
+int ocfd_get_time(ocfd_handle *h, int32_t *istep, double *tt) {
+  *istep = h->istep;
+  *tt = h->tt;
+  return 0;
+}
+
+// ---- one reference routine each ---------------------------------------------
+int ocfd_exchange_primitives(ocfd_handle *h) { return exchange_prims(h); }
+
+int ocfd_comput_rhybrid(ocfd_handle *h, int64_t counts[4]) {
+  if (!h->B.rhyb) return fail("comput_Rhybrid needs the HYBRID scheme");
+  if (sensor(h)) return -1;
+  if (counts) {
+    HIPCHK(hipMemsetAsync(h->counts, 0, 4 * sizeof(unsigned long long), h->st));
+    if (launch(h, "hybrid_counts", [&] { return launch_hybrid_counts(h->B, h->counts, h->st); })) return -1;
+    unsigned long long c[3];
+    HIPCHK(hipMemcpyAsync(c, h->counts, sizeof c, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int q = 0; q < 3; q++) counts[q] = (int64_t)c[q];
+    counts[3] = (int64_t)(h->cfg.nx + 1) * h->cfg.ny * h->cfg.nz;
+  }
+  return 0;
+}
+
+int ocfd_du_inviscous(ocfd_handle *h) { return inviscid(h, false); }
+int ocfd_du_inviscous_character(ocfd_handle *h) { return inviscid(h, true); }
+
+int ocfd_du_viscous(ocfd_handle *h) {
+  if (!h->B.ev) return fail("du_viscous needs IF_Viscous = 1");
+  if (launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
+  return launch(h, "visc_div", [&] { return launch_visc_div(h->B, h->st); });
+}
+
+int ocfd_time_adv_rk3(ocfd_handle *h, int32_t krk) {
+  if (krk < 1 || krk > 3) return fail("KRK must be 1, 2 or 3");
+  return launch(h, "rk3", [&] { return launch_rk3(h->B, krk, h->st); });
+}
+
+int ocfd_save_fn(ocfd_handle *h) { return launch(h, "copy_fn", [&] { return launch_copy_f(h->B, h->st); }); }
+
+int ocfd_comput_duvwt(ocfd_handle *h) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  if (launch(h, "duvwt", [&] { return launch_duvwt(h->B, h->neg, h->st); })) return -1;
+  if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+  int c[3];
+  if (read_neg(h, c)) return -1;
+  return c[1];  // cells with T <= 0 found (and repaired) in this call
+}
+
+int ocfd_bc(ocfd_handle *h) {
+  return launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); });
+}
+
+int ocfd_filtering(ocfd_handle *h) { return filtering(h); }
+
+int ocfd_advance_clock(ocfd_handle *h) {
+  h->istep += 1;
+  h->tt = h->tt + h->cfg.dt;
+  return 0;
+}
+
+// ---- whole steps -----------------------------------------------------------
+int ocfd_step(ocfd_handle *h, int32_t nsteps) {
+  const ocfd_config &c = h->cfg;
+  const bool hyb = c.scheme_invis == SCH_HYBRID;
+  const bool ch = c.if_scheme_character != 0;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  for (int n = 0; n < nsteps; n++) {
+    for (int krk = 1; krk <= 3; krk++) {
+      if (exchange_prims(h)) return -1;
+      if (krk == 1 && hyb && sensor(h)) return -1;
+      if (inviscid(h, ch)) return -1;
+      if (c.if_viscous == 1 && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
+      Block B = h->B;
+      if (c.if_viscous != 1) B.ev = nullptr;
+      const int refresh = h->ibc == 0 ? 1 : 0;
+      if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
+        return -1;
+      if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+      if (h->ibc != 0) {
+        if (launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); })) return -1;
+      } else {
+        bool any = false;
+        for (int q = 0; q < 6; q++) any |= (h->ghost[q] == 1 || h->ghost[q] == 2);
+        if (any && launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, h->ghost, 0, h->st); })) return -1;
+      }
+    }
+    h->istep += 1;
+    h->tt = h->tt + c.dt;
+    if (filtering(h)) return -1;
+  }
+  int cnt[3];
+  if (read_neg(h, cnt)) return -1;
+  if (cnt[2] > 10) return fail("more than 10 cells with T <= 0 in one stage");
+  return cnt[1];
+}
+
+// ---- diagnostics -------------------------------------------------------------
+int ocfd_show_flow_msg(ocfd_handle *h, double out[3]) {
+  if (launch(h, "flow_msg", [&] { return launch_reduce_flow_msg(h->B, h->work, h->wout, h->st); })) return -1;
+  double s[3];
+  HIPCHK(hipMemcpyAsync(s, h->wout, sizeof s, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const double n = (double)(h->cfg.nx * h->cfg.ny * h->cfg.nz);
+  for (int q = 0; q < 3; q++) out[q] = s[q] / n;
+  return 0;
+}
+
+int ocfd_ana_box(ocfd_handle *h, double out[20]) {
+  double *uvw[3] = {h->B.u, h->B.v, h->B.w};
+  if (launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, uvw, 3, h->st); })) return -1;
+  if (launch(h, "ana_box", [&] { return launch_ana_box(h->B, nullptr, nullptr, h->work, h->wout, h->st); }))
+    return -1;
+  double s[15];
+  HIPCHK(hipMemcpyAsync(s, h->wout, sizeof s, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  // statistics of ana_box (OCFD_ANA.f90:255-293)
+  const double n = (double)(h->cfg.nx * h->cfg.ny * h->cfg.nz);
+  for (int q = 0; q < 14; q++) s[q] = s[q] / n;
+  double u_rms = s[0], v_rms = s[1], w_rms = s[2], Ux_rms = s[3];
+  double Ux_sk = s[4], Ux_fl = s[5], d_aver = s[6], c_aver = s[7], Amu_aver = s[8], E_k = s[9];
+  double T_aver = s[10], u_sk = s[11], u_fl = s[12], ens = 0.5 * s[13];
+  double VV_rms = std::sqrt((u_rms + v_rms + w_rms) / 3.0);
+  double Amt = std::sqrt(u_rms + v_rms + w_rms) / c_aver;
+  u_rms = std::sqrt(u_rms); v_rms = std::sqrt(v_rms); w_rms = std::sqrt(w_rms); Ux_rms = std::sqrt(Ux_rms);
+  double Alamda = VV_rms / Ux_rms, Alamdax = u_rms / Ux_rms;
+  double Re_l = VV_rms * Alamda * d_aver / Amu_aver, Re_lx = VV_rms * Alamdax * d_aver / Amu_aver;
+  Ux_sk = Ux_sk / (Ux_rms * Ux_rms * Ux_rms);
+  Ux_fl = Ux_fl / ((Ux_rms * Ux_rms) * (Ux_rms * Ux_rms));
+  u_sk = u_sk / (u_rms * u_rms * u_rms);
+  u_fl = u_fl / ((u_rms * u_rms) * (u_rms * u_rms));
+  const double d_rms = std::sqrt(s[14] / n);
+  const double o[20] = {h->tt, Re_l, Re_lx, Amt, VV_rms, u_rms, v_rms, w_rms, u_sk, u_fl, Ux_sk, Ux_fl,
+                        E_k, ens, d_rms, T_aver, Amu_aver, d_aver, Ux_rms, Alamdax};
+  std::memcpy(out, o, sizeof o);
+  return 0;
+}
+
+// ---- instrumentation -------------------------------------------------------
+int ocfd_profile_enable(ocfd_handle *h, int32_t on) {
+  if (flush_prof(h)) return -1;
+  h->prof = on != 0;
+  return 0;
+}
+
+int ocfd_profile_read(ocfd_handle *h, int32_t max_entries, char names[][32], double *ms, int64_t *launches) {
+  if (flush_prof(h)) return -1;
+  int n = 0;
+  for (auto &kv : h->acc) {
+    if (n >= max_entries) break;
+    std::snprintf(names[n], 32, "%s", kv.first.c_str());
+    ms[n] = kv.second.first;
+    launches[n] = kv.second.second;
+    n++;
+  }
+  return n;
+}
+
+int ocfd_profile_reset(ocfd_handle *h) {
+  if (flush_prof(h)) return -1;
+  h->acc.clear();
+  return 0;
+}
+
+}  // extern "C"

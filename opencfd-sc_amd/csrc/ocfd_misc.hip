@@ -1,0 +1,818 @@
+// ocfd_misc.hip -- element-wise and small kernels of the step:
+//   periodic halo fill        exchange_boundary_x/y/z on one block (OCFD_mpi.f90:198-233)
+//   shock sensor              comput_Rhybrid (OCFD_SetHybrid.f90:222-357)
+//   RK3 / primitives          OCFD_time_adv_RK3, comput_duvwT (OCFD_Time_Adv.f90:4-63)
+//   boundary conditions       OCFD_bc + Flow_Ghost_Extent_2nd (OCFD_BC.f90:2-204,
+//                             OCFD_GhostCell.f90:164-286)
+//   filters                   filter_[xyz]3d[_shock] (OCFD_filtering.f90:73-363)
+//   diagnostics               show_flow_msg sums, ana_box sums (two-level,
+//                             fixed-order reductions: run-to-run deterministic)
+//   grid metrics              init_mesh (OCFD_mesh.f90:18-107, OCFD_GhostCell.f90)
+#include "ocfd_kernels.h"
+
+namespace ocfd {
+
+namespace {
+constexpr int NT = 256;
+inline unsigned nb(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
+
+__device__ inline void cell_of(const Geom &g, int64_t t, int &i, int &j, int &k) {
+  i = (int)(t % g.nx) + 1;
+  int64_t r = t / g.nx;
+  j = (int)(r % g.ny) + 1;
+  k = (int)(r / g.ny) + 1;
+}
+
+// ---------------------------------------------------------------------------
+// periodic halo fill: for every periodic axis copy interior planes 1..L to
+// n+1..n+L and n-L+1..n to 1-L..0, over the interior range of the other two
+// indices (the extent of TYPE_LAP*2, OCFD_mpi.f90:114-122).
+// ---------------------------------------------------------------------------
+struct HaloFields {
+  double *p[8];
+};
+__global__ void k_halo(const Geom g, const HaloFields F, const int nf, const int px, const int py, const int pz) {
+  const int64_t nxh = px ? (int64_t)2 * LAP * g.ny * g.nz : 0;
+  const int64_t nyh = py ? (int64_t)2 * LAP * g.nx * g.nz : 0;
+  const int64_t nzh = pz ? (int64_t)2 * LAP * g.nx * g.ny : 0;
+  const int64_t tot = nxh + nyh + nzh;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  int64_t dst, src;
+  if (t < nxh) {
+    int64_t r = t;
+    const int l = (int)(r % (2 * LAP)); r /= 2 * LAP;
+    const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+    if (l < LAP) { dst = g.hi(g.nx + 1 + l, j, k); src = g.hi(1 + l, j, k); }
+    else { const int q = l - LAP; dst = g.hi(1 - LAP + q, j, k); src = g.hi(g.nx - LAP + 1 + q, j, k); }
+  } else if (t < nxh + nyh) {
+    int64_t r = t - nxh;
+    const int i = (int)(r % g.nx) + 1; r /= g.nx;
+    const int l = (int)(r % (2 * LAP)); r /= 2 * LAP;
+    const int k = (int)r + 1;
+    if (l < LAP) { dst = g.hi(i, g.ny + 1 + l, k); src = g.hi(i, 1 + l, k); }
+    else { const int q = l - LAP; dst = g.hi(i, 1 - LAP + q, k); src = g.hi(i, g.ny - LAP + 1 + q, k); }
+  } else {
+    int64_t r = t - nxh - nyh;
+    const int i = (int)(r % g.nx) + 1; r /= g.nx;
+    const int j = (int)(r % g.ny) + 1; r /= g.ny;
+    const int l = (int)r;
+    if (l < LAP) { dst = g.hi(i, j, g.nz + 1 + l); src = g.hi(i, j, 1 + l); }
+    else { const int q = l - LAP; dst = g.hi(i, j, 1 - LAP + q); src = g.hi(i, j, g.nz - LAP + 1 + q); }
+  }
+  for (int q = 0; q < nf; q++) F.p[q][dst] = F.p[q][src];
+}
+
+// ---------------------------------------------------------------------------
+// shock sensor 1 (OCFD_SetHybrid.f90:240-257): R = sum over axes of
+// |p(-1) - 2p + p(+1)| / |p(-1) + 2p + p(+1)|, p = p00 d T
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_sensor1(const Block B, const double p00) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int i, j, k;
+  cell_of(g, t, i, j, k);
+  const int64_t h = g.hi(i, j, k);
+  auto P = [&](int64_t q) { return p00 * B.d[q] * B.T[q]; };
+  const double pc = P(h);
+  double r = 0.0;
+  double acc[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int64_t st = g.hstride(a);
+    const double pm = P(h - st), pp = P(h + st);
+    acc[a] = fabs((pm - 2.0 * pc + pp) / (pm + 2.0 * pc + pp));
+  }
+  r = acc[0] + acc[1] + acc[2];
+  B.rhyb[h] = r;
+}
+
+// sensor 2 (OCFD_SetHybrid.f90:259-290), pass 1: |grad p| and block sums
+__global__ __launch_bounds__(NT) void k_sensor2a(const Block B, const double p00, double *partial) {
+  const Geom &g = B.g;
+  __shared__ double red[NT];
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  double r = 0.0;
+  if (t < g.ni) {
+    int i, j, k;
+    cell_of(g, t, i, j, k);
+    const int64_t h = g.hi(i, j, k);
+    double dp[3];
+    for (int a = 0; a < 3; a++) {
+      const int64_t st = g.hstride(a);
+      const int s = a == 0 ? i : (a == 1 ? j : k);
+      auto at = [&](int q) { const int64_t x = h + (int64_t)(q - s) * st; return p00 * B.d[x] * B.T[x]; };
+      dp[a] = dxyz0_point(at, s, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+    }
+    const Metrics &M = B.M;
+    const double px = dp[0] * M.get(M_AKX, h) + dp[1] * M.get(M_AIX, h) + dp[2] * M.get(M_ASX, h);
+    const double py = dp[0] * M.get(M_AKY, h) + dp[1] * M.get(M_AIY, h) + dp[2] * M.get(M_ASY, h);
+    const double pz = dp[0] * M.get(M_AKZ, h) + dp[1] * M.get(M_AIZ, h) + dp[2] * M.get(M_ASZ, h);
+    r = sqrt(px * px + py * py + pz * pz);
+    B.rhyb[h] = r;
+  }
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// fixed-order sum of `n` partials by one block into out[0]
+__global__ __launch_bounds__(NT) void k_sum_partials(const double *partial, int64_t n, int stride, int nq,
+                                                     double *out) {
+  __shared__ double red[NT];
+  for (int q = 0; q < nq; q++) {
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < n; b += NT) s += partial[b * stride + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+    __syncthreads();
+  }
+}
+
+// sensor 2, pass 2: R = R / (mean + 1e-30)
+__global__ __launch_bounds__(NT) void k_sensor2b(const Block B, const double *sum) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int i, j, k;
+  cell_of(g, t, i, j, k);
+  const double dp_av = sum[0] / (1.0 * g.nx * g.ny * g.nz);
+  const double dp_av1 = 1.0 / (dp_av + 1.0e-30);
+  const int64_t h = g.hi(i, j, k);
+  B.rhyb[h] = B.rhyb[h] * dp_av1;
+}
+
+// scheme shares over x-faces (OCFD_SetHybrid.f90:320-336)
+__global__ __launch_bounds__(NT) void k_hyb_counts(const Block B, unsigned long long *counts) {
+  const Geom &g = B.g;
+  const int64_t nf = (int64_t)(g.nx + 1) * g.ny * g.nz;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  int sc = 0;
+  if (t < nf) {
+    const int i = (int)(t % (g.nx + 1));
+    const int64_t r = t / (g.nx + 1);
+    const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+    const double r0 = dmax2(B.rhyb[g.hi(i, j, k)], B.rhyb[g.hi(i + 1, j, k)]);
+    sc = hybrid_index(r0, B.S.hyb2, B.S.hyb3);
+  }
+  __shared__ unsigned cnt[3];
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  if (sc) atomicAdd(&cnt[sc - 1], 1u);
+  __syncthreads();
+  if (threadIdx.x < 3 && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// RK3 stage (OCFD_Time_Adv.f90:4-36) / fn = f / init3d f / comput_duvwT
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_rk3(const Block B, const int krk) {
+  const int64_t n5 = 5 * B.g.ni;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= n5) return;
+  const double dt = B.P.dt;
+  if (krk == 1) {
+    B.f[t] = B.fn[t] + dt * B.du[t];
+  } else {
+    const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
+    const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
+    B.f[t] = ra * B.fn[t] + rb * (B.f[t] + dt * B.du[t]);
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_copy(const double *src, double *dst, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// f from primitives (init3d, OCFD_init.f90:10-20)
+__global__ __launch_bounds__(NT) void k_init_f(const Block B) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int i, j, k;
+  cell_of(g, t, i, j, k);
+  const int64_t h = g.hi(i, j, k);
+  const double d = B.d[h], u = B.u[h], v = B.v[h], w = B.w[h];
+  const int64_t ni = g.ni;
+  B.f[t] = d;
+  B.f[t + ni] = d * u;
+  B.f[t + 2 * ni] = d * v;
+  B.f[t + 3 * ni] = d * w;
+  B.f[t + 4 * ni] = d * ((u * u + v * v + w * w) * 0.5 + B.P.cv * B.T[h]);
+}
+
+__global__ __launch_bounds__(NT) void k_duvwt(const Block B, int *neg) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int i, j, k;
+  cell_of(g, t, i, j, k);
+  const int64_t h = g.hi(i, j, k), ni = g.ni;
+  const double f1 = B.f[t], f2 = B.f[t + ni], f3 = B.f[t + 2 * ni], f4 = B.f[t + 3 * ni], f5 = B.f[t + 4 * ni];
+  const double u = f2 / f1, v = f3 / f1, w = f4 / f1;
+  const double T = (f5 - (f2 * u + f3 * v + f4 * w) * 0.5) / (f1 * B.P.cv);
+  B.d[h] = f1; B.u[h] = u; B.v[h] = v; B.w[h] = w; B.T[h] = T;
+  if (T <= 0) {
+    const int slot = atomicAdd(neg, 1);
+    if (slot < 64) neg[1 + slot] = (int)t;
+  }
+}
+
+// T <= 0 repair (OCFD_handle_NegativeT.f90:11): T = mean of the six
+// neighbours; the listed cells are processed in ascending index order by one
+// thread.  Neighbour values are those after the whole primitive update.
+__global__ void k_negt_repair(const Block B, int *neg, const int refresh) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int cnt = neg[0];
+  if (cnt <= 0) return;
+  if (cnt > 64) cnt = 64;
+  int *lst = neg + 1;
+  for (int a = 1; a < cnt; a++) {  // insertion sort of <= 64 entries
+    int x = lst[a], b = a - 1;
+    while (b >= 0 && lst[b] > x) { lst[b + 1] = lst[b]; b--; }
+    lst[b + 1] = x;
+  }
+  const Geom &g = B.g;
+  for (int a = 0; a < cnt; a++) {
+    int i, j, k;
+    cell_of(g, lst[a], i, j, k);
+    const int64_t h = g.hi(i, j, k);
+    B.T[h] = (B.T[g.hi(i + 1, j, k)] + B.T[g.hi(i - 1, j, k)] + B.T[g.hi(i, j + 1, k)] + B.T[g.hi(i, j - 1, k)] +
+              B.T[g.hi(i, j, k + 1)] + B.T[g.hi(i, j, k - 1)]) / 6.0;
+    if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz)) {
+      const int64_t ni = g.ni, q = lst[a];
+      const double d = B.d[h], u = B.u[h], v = B.v[h], w = B.w[h];
+      B.f[q + 4 * ni] = d * (B.P.cv * B.T[h] + (u * u + v * v + w * w) * 0.5);
+    }
+  }
+  const int total = neg[0];
+  neg[65] += total;
+  if (total > neg[66]) neg[66] = total;
+  neg[0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions
+// ---------------------------------------------------------------------------
+// BC hooks on the j=1 / j=ny planes: ocfd_bc_blunt2d (OCFD_BC_Def.f90:4-33),
+// or the isothermal/adiabatic wall + free stream of the BC_USER_DEF hook.
+__global__ __launch_bounds__(NT) void k_bc_hook(const Block B, const int ibc, const double Tw) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.nz) return;
+  const int i = (int)(t % g.nx) + 1, k = (int)(t / g.nx) + 1;
+  const int64_t h1 = g.hi(i, 1, k), h2 = g.hi(i, 2, k), h3 = g.hi(i, 3, k), hn = g.hi(i, g.ny, k);
+  if (ibc == 1) {
+    B.d[h1] = B.d[h2]; B.u[h1] = 0.0; B.v[h1] = 0.0; B.w[h1] = 0.0; B.T[h1] = B.T[h2];
+  } else {
+    B.u[h1] = 0.0; B.v[h1] = 0.0; B.w[h1] = 0.0;
+    if (Tw > 0.0) {
+      B.d[h1] = (4.0 * B.d[h2] * B.T[h2] - B.d[h3] * B.T[h3]) / (3.0 * Tw);
+      B.T[h1] = Tw;
+    } else {
+      B.d[h1] = (4.0 * B.d[h2] - B.d[h3]) / 3.0;
+      B.T[h1] = (4.0 * B.T[h2] - B.T[h3]) / 3.0;
+    }
+  }
+  B.d[hn] = 1.0; B.u[hn] = 1.0; B.v[hn] = 0.0; B.w[hn] = 0.0; B.T[hn] = 1.0;
+}
+
+// f recomputed from the primitives on the six block faces (OCFD_BC.f90:132-200)
+__global__ __launch_bounds__(NT) void k_bc_planes(const Block B) {
+  const Geom &g = B.g;
+  const int64_t ax = (int64_t)g.ny * g.nz, ay = (int64_t)g.nx * g.nz, az = (int64_t)g.nx * g.ny;
+  int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= 2 * (ax + ay + az)) return;
+  int i, j, k;
+  if (t < 2 * ax) {
+    const int side = (int)(t / ax); const int64_t r = t % ax;
+    j = (int)(r % g.ny) + 1; k = (int)(r / g.ny) + 1; i = side ? g.nx : 1;
+  } else if (t < 2 * (ax + ay)) {
+    t -= 2 * ax;
+    const int side = (int)(t / ay); const int64_t r = t % ay;
+    i = (int)(r % g.nx) + 1; k = (int)(r / g.nx) + 1; j = side ? g.ny : 1;
+  } else {
+    t -= 2 * (ax + ay);
+    const int side = (int)(t / az); const int64_t r = t % az;
+    i = (int)(r % g.nx) + 1; j = (int)(r / g.nx) + 1; k = side ? g.nz : 1;
+  }
+  const int64_t h = g.hi(i, j, k), q = g.ii(i, j, k), ni = g.ni;
+  const double d = B.d[h], u = B.u[h], v = B.v[h], w = B.w[h];
+  B.f[q] = d;
+  B.f[q + ni] = d * u;
+  B.f[q + 2 * ni] = d * v;
+  B.f[q + 3 * ni] = d * w;
+  B.f[q + 4 * ni] = d * (B.P.cv * B.T[h] + (u * u + v * v + w * w) * 0.5);
+}
+
+// Flow_Ghost_Extent_2nd (OCFD_GhostCell.f90:164-286)
+__global__ __launch_bounds__(NT) void k_ghost(const Block B, const int side) {
+  const Geom &g = B.g;
+  const int axis = side / 2, hi = side % 2;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;  // the two other axes
+  const int n1 = g.n(a1), n2 = g.n(a2), n = g.n(axis);
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)LAP * n1 * n2) return;
+  const int l = (int)(t % LAP);
+  const int64_t r = t / LAP;
+  int idx[3], wi[3], mi[3];
+  idx[a1] = (int)(r % n1) + 1;
+  idx[a2] = (int)(r / n1) + 1;
+  const int s = hi ? n + 1 + l : 1 - LAP + l;
+  idx[axis] = s;
+  for (int q = 0; q < 3; q++) { wi[q] = idx[q]; mi[q] = idx[q]; }
+  wi[axis] = hi ? n : 1;
+  mi[axis] = hi ? 2 * n - s : 2 - s;
+  const int64_t gq = g.hi(idx[0], idx[1], idx[2]), wq = g.hi(wi[0], wi[1], wi[2]), mq = g.hi(mi[0], mi[1], mi[2]);
+  const double s1 = 0.8, s2 = 1.2;
+  B.u[gq] = 2.0 * B.u[wq] - B.u[mq];
+  B.v[gq] = 2.0 * B.v[wq] - B.v[mq];
+  B.w[gq] = 2.0 * B.w[wq] - B.w[mq];
+  double d = 2.0 * B.d[wq] - B.d[mq];
+  double T = 2.0 * B.T[wq] - B.T[mq];
+  if (d < s1 * B.d[wq]) d = s1 * B.d[wq];
+  if (d > s2 * B.d[wq]) d = s2 * B.d[wq];
+  if (T < s1 * B.T[wq]) T = s1 * B.T[wq];
+  if (T > s2 * B.T[wq]) T = s2 * B.T[wq];
+  B.d[gq] = d;
+  B.T[gq] = T;
+}
+
+// ---------------------------------------------------------------------------
+// filters: out-of-place along one axis; cells outside the filtered index box
+// are copied unchanged
+// ---------------------------------------------------------------------------
+struct FilterArgs {
+  int axis, scheme;
+  double s0, rth;
+  int lo[3], hi[3];
+};
+__global__ __launch_bounds__(NT) void k_filter(const Block B, const FilterArgs A, double *fout) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int idx[3];
+  cell_of(g, t, idx[0], idx[1], idx[2]);
+  const int axis = A.axis, n = g.n(axis);
+  const int s = idx[axis];
+  const int64_t st = g.istride(axis), ni = g.ni;
+  const bool in = idx[0] >= A.lo[0] && idx[0] <= A.hi[0] && idx[1] >= A.lo[1] && idx[1] <= A.hi[1] &&
+                  idx[2] >= A.lo[2] && idx[2] <= A.hi[2];
+  auto wrap = [&](int q) { return q < 1 ? q + n : (q > n ? q - n : q); };
+  for (int m = 0; m < 5; m++) {
+    const double *f0 = B.f + (int64_t)m * ni;
+    double r = f0[t];
+    if (in) {
+      auto G = [&](int o) { return f0[t + (int64_t)(wrap(s + o) - s) * st]; };
+      if (A.scheme == 1) {
+        // Bogey-Bailly Fo9p (OCFD_filtering.f90:80-81, 101-102)
+        const double d0 = 0.243527493120, d1 = -0.204788880640, d2 = 0.120007591680, d3 = -0.045211119360,
+                     d4 = 0.008228661760;
+        r = G(0) - A.s0 * (d0 * G(0) + d1 * (G(-1) + G(1)) + d2 * (G(-2) + G(2)) + d3 * (G(-3) + G(3)) +
+                           d4 * (G(-4) + G(4)));
+      } else {
+        // shock-capturing filter (OCFD_filtering.f90:195-239)
+        const double c1 = -0.210383, c2 = 0.039617, gm1 = B.P.gamma - 1.0;
+        auto P = [&](int o) {
+          const int64_t q = t + (int64_t)(wrap(s + o) - s) * st;
+          const double f1 = B.f[q], f2 = B.f[q + ni], f3 = B.f[q + 2 * ni], f4 = B.f[q + 3 * ni], f5 = B.f[q + 4 * ni];
+          return (f5 - 0.5 * (f2 * f2 + f3 * f3 + f4 * f4) / f1) * gm1;
+        };
+        double pv[7];
+        for (int o = -3; o <= 3; o++) pv[o + 3] = P(o);
+        double sc0[3];
+        for (int c = -1; c <= 1; c++) {
+          const double *pp = pv + 3 + c;  // pp[0] = p(s+c)
+          const double dp0 = 0.25 * (-pp[1] + 2.0 * pp[0] - pp[-1]);
+          const double dp1 = 0.25 * (-pp[2] + 2.0 * pp[1] - pp[0]);
+          const double dp2 = 0.25 * (-pp[0] + 2.0 * pp[-1] - pp[-2]);
+          const double x = dp0 - dp1, y = dp0 - dp2;
+          const double ri = 0.5 * (x * x + y * y) / (pp[0] * pp[0]) + 1.0e-16;
+          sc0[c + 1] = 0.5 * (1.0 - A.rth / ri + fabs(1.0 - A.rth / ri));
+        }
+        const double sc1 = 0.5 * (sc0[1] + sc0[2]);
+        const double sc2 = 0.5 * (sc0[1] + sc0[0]);
+        r = G(0) - A.s0 * (sc1 * (c1 * (G(1) - G(0)) + c2 * (G(2) - G(-1))) -
+                           sc2 * (c1 * (G(0) - G(-1)) + c2 * (G(1) - G(-2))));
+      }
+    }
+    fout[t + (int64_t)m * ni] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+// show_flow_msg sums (OCFD_Time_Adv.f90:77-86): 3 per block
+__global__ __launch_bounds__(NT) void k_flow_msg(const Block B, double *partial) {
+  const Geom &g = B.g;
+  __shared__ double red[3][NT];
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  double e[3] = {0.0, 0.0, 0.0};
+  if (t < g.ni) {
+    int i, j, k;
+    cell_of(g, t, i, j, k);
+    const int64_t h = g.hi(i, j, k);
+    const double p00 = 1.0 / (B.P.gamma * B.P.ma * B.P.ma), tmp1 = 1.0 - B.P.gamma;
+    const double tmp0 = 1.0 / B.M.get(M_AJAC, h);
+    const double d = B.d[h], u = B.u[h], v = B.v[h], w = B.w[h];
+    e[0] = B.f[t + 4 * g.ni] * tmp0;
+    e[1] = d * (u * u + v * v + w * w) * 0.5 * tmp0;
+    e[2] = p00 * pow(d, tmp1) * B.T[h] * tmp0;
+  }
+  for (int q = 0; q < 3; q++) red[q][threadIdx.x] = e[q];
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; q++) partial[blockIdx.x * 3 + q] = red[q][0];
+}
+
+// ana_box sums (OCFD_ANA.f90:215-243): 14 per block; pass 2 (dmean != null):
+// sum of (d - dmean)^2
+__global__ __launch_bounds__(NT) void k_ana_box(const Block B, double *partial, const double *dsum) {
+  const Geom &g = B.g;
+  __shared__ double red[NT];
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  double s[14];
+  for (int q = 0; q < 14; q++) s[q] = 0.0;
+  const int nq = dsum ? 1 : 14;
+  if (t < g.ni) {
+    int i, j, k;
+    cell_of(g, t, i, j, k);
+    const int64_t h = g.hi(i, j, k);
+    if (dsum) {
+      const double d_aver = dsum[0] / (double)(g.nx * g.ny * g.nz);
+      const double x = B.d[h] - d_aver;
+      s[0] = x * x;
+    } else {
+      double gr[9];
+      const double *fld[3] = {B.u, B.v, B.w};
+      for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) {
+          const int64_t st = g.hstride(a);
+          const int sidx = a == 0 ? i : (a == 1 ? j : k);
+          const double *arr = fld[c];
+          auto at = [&](int q) { return arr[h + (int64_t)(q - sidx) * st]; };
+          gr[c * 3 + a] =
+              dxyz0_point(at, sidx, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+        }
+      const Metrics &M = B.M;
+      auto X = [&](int c) { return gr[c * 3] * M.get(M_AKX, h) + gr[c * 3 + 1] * M.get(M_AIX, h) + gr[c * 3 + 2] * M.get(M_ASX, h); };
+      auto Y = [&](int c) { return gr[c * 3] * M.get(M_AKY, h) + gr[c * 3 + 1] * M.get(M_AIY, h) + gr[c * 3 + 2] * M.get(M_ASY, h); };
+      auto Z = [&](int c) { return gr[c * 3] * M.get(M_AKZ, h) + gr[c * 3 + 1] * M.get(M_AIZ, h) + gr[c * 3 + 2] * M.get(M_ASZ, h); };
+      const double ux = X(0), uy = Y(0), uz = Z(0), vx = X(1), vz = Z(1), wx = X(2), wy = Y(2);
+      const double u = B.u[h], v = B.v[h], w = B.w[h], d = B.d[h], T = B.T[h];
+      const double Tsb = 110.4 / B.P.ref_amu_t0, TmpR = 1.0 / B.P.re;
+      const double amu = TmpR * (1.0 + Tsb) * sqrt(T * T * T) / (Tsb + T);
+      s[0] = u * u; s[1] = v * v; s[2] = w * w; s[3] = ux * ux;
+      s[4] = ux * ux * ux; s[5] = (ux * ux) * (ux * ux);
+      s[6] = d; s[7] = sqrt(T) / B.P.ma; s[8] = amu;
+      s[9] = (u * u + v * v + w * w) * d * 0.5; s[10] = T;
+      s[11] = u * u * u; s[12] = (u * u) * (u * u);
+      const double e1 = wy - vz, e2 = uz - wx, e3 = vx - uy;
+      s[13] = e1 * e1 + e2 * e2 + e3 * e3;
+    }
+  }
+  for (int q = 0; q < nq; q++) {
+    red[threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x * nq + q] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid metrics
+// ---------------------------------------------------------------------------
+// updata_Axyz_periodic (OCFD_mesh.f90:109-185)
+__global__ __launch_bounds__(NT) void k_coord_shift(const Geom g, double *X, double *Y, double *Z, const int axis,
+                                                    const double s0, const double s1, const double s2) {
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n1 = g.n(a1), n2 = g.n(a2), n = g.n(axis);
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)2 * LAP * n1 * n2) return;
+  const int l = (int)(t % (2 * LAP));
+  const int64_t r = t / (2 * LAP);
+  int idx[3];
+  idx[a1] = (int)(r % n1) + 1;
+  idx[a2] = (int)(r / n1) + 1;
+  double sg;
+  if (l < LAP) { idx[axis] = 1 - LAP + l; sg = -1.0; } else { idx[axis] = n + 1 + (l - LAP); sg = 1.0; }
+  const int64_t h = g.hi(idx[0], idx[1], idx[2]);
+  if (sg < 0) { X[h] = X[h] - s0; Y[h] = Y[h] - s1; Z[h] = Z[h] - s2; }
+  else { X[h] = X[h] + s0; Y[h] = Y[h] + s1; Z[h] = Z[h] + s2; }
+}
+
+__device__ inline void jac_point(double *const *met, int64_t h, double xi1, double xj1, double xk1, double yi1,
+                                 double yj1, double yk1, double zi1, double zj1, double zk1) {
+  const double Jac1 = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 -
+                             yi1 * xj1 * zk1 - xi1 * zj1 * yk1);
+  met[M_AJAC][h] = Jac1;
+  met[M_AKX][h] = Jac1 * (yj1 * zk1 - zj1 * yk1);
+  met[M_AKY][h] = Jac1 * (zj1 * xk1 - xj1 * zk1);
+  met[M_AKZ][h] = Jac1 * (xj1 * yk1 - yj1 * xk1);
+  met[M_AIX][h] = Jac1 * (yk1 * zi1 - zk1 * yi1);
+  met[M_AIY][h] = Jac1 * (zk1 * xi1 - xk1 * zi1);
+  met[M_AIZ][h] = Jac1 * (xk1 * yi1 - yk1 * xi1);
+  met[M_ASX][h] = Jac1 * (yi1 * zj1 - zi1 * yj1);
+  met[M_ASY][h] = Jac1 * (zi1 * xj1 - xi1 * zj1);
+  met[M_ASZ][h] = Jac1 * (xi1 * yj1 - yi1 * xj1);
+}
+
+struct MetPtrs {
+  double *p[M_N];
+};
+
+// comput_Jacobian3d (OCFD_mesh.f90:61-107) on the interior
+__global__ __launch_bounds__(NT) void k_jacobian(const Block B, const double *X, const double *Y, const double *Z,
+                                                 const MetPtrs mp) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.ni) return;
+  int i, j, k;
+  cell_of(g, t, i, j, k);
+  const int64_t h = g.hi(i, j, k);
+  double dd[9];
+  const double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; c++)
+    for (int a = 0; a < 3; a++) {
+      const int64_t st = g.hstride(a);
+      const int s = a == 0 ? i : (a == 1 ? j : k);
+      const double *arr = C[c];
+      auto at = [&](int q) { return arr[h + (int64_t)(q - s) * st]; };
+      dd[c * 3 + a] = dxyz0_point(at, s, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+    }
+  jac_point(mp.p, h, dd[0], dd[1], dd[2], dd[3], dd[4], dd[5], dd[6], dd[7], dd[8]);
+}
+
+// Jac_Ghost_Extent_1st (OCFD_GhostCell.f90:18-160)
+__global__ __launch_bounds__(NT) void k_jac_ghost1(const Geom g, const int side, double *X, double *Y, double *Z,
+                                                   const MetPtrs mp) {
+  const int axis = side / 2, hi = side % 2;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n1 = g.n(a1), n2 = g.n(a2), n = g.n(axis);
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)LAP * n1 * n2) return;
+  const int l = (int)(t % LAP);
+  const int64_t r = t / LAP;
+  int idx[3], w[3], w2[3];
+  idx[a1] = (int)(r % n1) + 1;
+  idx[a2] = (int)(r / n1) + 1;
+  const int s = hi ? n + 1 + l : 1 - LAP + l;
+  idx[axis] = s;
+  for (int q = 0; q < 3; q++) { w[q] = idx[q]; w2[q] = idx[q]; }
+  w[axis] = hi ? n : 1;
+  w2[axis] = hi ? n - 1 : 2;
+  const int64_t gq = g.hi(idx[0], idx[1], idx[2]), hw = g.hi(w[0], w[1], w[2]), hw2 = g.hi(w2[0], w2[1], w2[2]);
+  for (int m = 0; m <= M_AJAC; m++) mp.p[m][gq] = mp.p[m][hw];
+  const double fac = hi ? (double)(s - n) : (1.0 - s);
+  double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; c++) C[c][gq] = C[c][hw] + fac * (C[c][hw] - C[c][hw2]);
+}
+
+// Jac_Ghost_Extent_2nd, coordinates: odd reflection (OCFD_GhostCell.f90:294-378)
+__global__ __launch_bounds__(NT) void k_jac_ghost2_coords(const Geom g, const int side, double *X, double *Y,
+                                                          double *Z) {
+  const int axis = side / 2, hi = side % 2;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n1 = g.n(a1), n2 = g.n(a2), n = g.n(axis);
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)LAP * n1 * n2) return;
+  const int l = (int)(t % LAP);
+  const int64_t r = t / LAP;
+  int idx[3], w[3], mr[3];
+  idx[a1] = (int)(r % n1) + 1;
+  idx[a2] = (int)(r / n1) + 1;
+  const int s = hi ? n + 1 + l : 1 - LAP + l;
+  idx[axis] = s;
+  for (int q = 0; q < 3; q++) { w[q] = idx[q]; mr[q] = idx[q]; }
+  w[axis] = hi ? n : 1;
+  mr[axis] = hi ? 2 * n - s : 2 - s;
+  const int64_t gq = g.hi(idx[0], idx[1], idx[2]), hw = g.hi(w[0], w[1], w[2]), hm = g.hi(mr[0], mr[1], mr[2]);
+  double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; c++) C[c][gq] = 2.0 * C[c][hw] - C[c][hm];
+}
+
+// comput_Jacobian3d_Ghost (OCFD_GhostCell.f90:394-500)
+__device__ inline double ghost_deriv(const Geom &g, const double *A, int axis, int64_t h, int s, int lo, int hi,
+                                     double hh) {
+  const int64_t st = g.hstride(axis);
+  auto X = [&](int o) { return A[h + (int64_t)o * st]; };
+  if (s == lo) return (-(3.0 * X(0)) + 4.0 * X(1) - X(2)) / (2.0 * hh);
+  if (s == hi) return (X(-2) - 4.0 * X(-1) + 3.0 * X(0)) / (2.0 * hh);
+  if (s == lo + 1 || s == hi - 1) return (X(1) - X(-1)) / (2.0 * hh);
+  return (8.0 * (X(1) - X(-1)) - (X(2) - X(-2))) / (12.0 * hh);
+}
+__global__ __launch_bounds__(NT) void k_jac_ghost2(const Block B, const int side, const double *X, const double *Y,
+                                                   const double *Z, const MetPtrs mp) {
+  const Geom &g = B.g;
+  const int axis = side / 2, hi = side % 2;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n1 = g.n(a1), n2 = g.n(a2), n = g.n(axis);
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)LAP * n1 * n2) return;
+  const int l = (int)(t % LAP);
+  const int64_t r = t / LAP;
+  int idx[3];
+  idx[a1] = (int)(r % n1) + 1;
+  idx[a2] = (int)(r / n1) + 1;
+  idx[axis] = hi ? n + 1 + l : 1 - LAP + l;
+  int b1[3] = {1, 1, 1}, e1[3] = {g.nx, g.ny, g.nz};
+  if (hi) { b1[axis] = n; e1[axis] = n + LAP; } else { b1[axis] = 1 - LAP; e1[axis] = 1; }
+  const int64_t h = g.hi(idx[0], idx[1], idx[2]);
+  const double *C[3] = {X, Y, Z};
+  double dd[3][3];
+  for (int c = 0; c < 3; c++)
+    for (int a = 0; a < 3; a++) dd[c][a] = ghost_deriv(g, C[c], a, h, idx[a], b1[a], e1[a], B.P.h[a]);
+  jac_point(mp.p, h, dd[0][0], dd[0][1], dd[0][2], dd[1][0], dd[1][1], dd[1][2], dd[2][0], dd[2][1], dd[2][2]);
+}
+
+// Akx1 = Akx/Ajac ... over the whole halo'd block (OCFD_mesh.f90:41-55)
+__global__ __launch_bounds__(NT) void k_normalise(const Geom g, const MetPtrs mp) {
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= g.nh) return;
+  const double J = mp.p[M_AJAC][t];
+  for (int m = 0; m < 9; m++) mp.p[M_AKX1 + m][t] = mp.p[M_AKX + m][t] / J;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st) {
+  const Geom &g = B.g;
+  const int px = B.S.iperiodic[0] == 1, py = B.S.iperiodic[1] == 1, pz = B.S.iperiodic[2] == 1;
+  const int64_t tot = (px ? (int64_t)2 * LAP * g.ny * g.nz : 0) + (py ? (int64_t)2 * LAP * g.nx * g.nz : 0) +
+                      (pz ? (int64_t)2 * LAP * g.nx * g.ny : 0);
+  if (tot == 0) return hipSuccess;
+  for (int f0 = 0; f0 < nfields; f0 += 8) {
+    HaloFields F;
+    const int nf = nfields - f0 < 8 ? nfields - f0 : 8;
+    for (int q = 0; q < nf; q++) F.p[q] = fields[f0 + q];
+    hipLaunchKernelGGL(k_halo, dim3(nb(tot)), dim3(NT), 0, st, g, F, nf, px, py, pz);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sensor(const Block &B, hipStream_t st) {
+  const double p00 = 1.0 / (B.P.gamma * B.P.ma * B.P.ma);
+  hipLaunchKernelGGL(k_sensor1, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, p00);
+  return hipGetLastError();
+}
+
+hipError_t launch_sensor2(const Block &B, double *work, hipStream_t st) {
+  const double p00 = 1.0 / (B.P.gamma * B.P.ma * B.P.ma);
+  const unsigned nblk = nb(B.g.ni);
+  double *partial = work, *sum = work + nblk;
+  hipLaunchKernelGGL(k_sensor2a, dim3(nblk), dim3(NT), 0, st, B, p00, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(NT), 0, st, partial, (int64_t)nblk, 1, 1, sum);
+  hipLaunchKernelGGL(k_sensor2b, dim3(nblk), dim3(NT), 0, st, B, sum);
+  return hipGetLastError();
+}
+
+hipError_t launch_hybrid_counts(const Block &B, unsigned long long *counts, hipStream_t st) {
+  const int64_t nf = (int64_t)(B.g.nx + 1) * B.g.ny * B.g.nz;
+  hipLaunchKernelGGL(k_hyb_counts, dim3(nb(nf)), dim3(NT), 0, st, B, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk3(const Block &B, int krk, hipStream_t st) {
+  hipLaunchKernelGGL(k_rk3, dim3(nb(5 * B.g.ni)), dim3(NT), 0, st, B, krk);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_f(const Block &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy, dim3(nb(5 * B.g.ni)), dim3(NT), 0, st, (const double *)B.f, B.fn, 5 * B.g.ni);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_f(const Block &B, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_f, dim3(nb(B.g.ni)), dim3(NT), 0, st, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st) {
+  hipLaunchKernelGGL(k_duvwt, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, neg);
+  return hipGetLastError();
+}
+
+hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st) {
+  hipLaunchKernelGGL(k_negt_repair, dim3(1), dim3(64), 0, st, B, neg, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(const Block &B, int ibc, double tw, const int *ghost, int planes, hipStream_t st) {
+  const Geom &g = B.g;
+  if (ibc == 1 || ibc == 99) hipLaunchKernelGGL(k_bc_hook, dim3(nb((int64_t)g.nx * g.nz)), dim3(NT), 0, st, B, ibc, tw);
+  const int64_t np = 2 * ((int64_t)g.ny * g.nz + (int64_t)g.nx * g.nz + (int64_t)g.nx * g.ny);
+  if (planes) hipLaunchKernelGGL(k_bc_planes, dim3(nb(np)), dim3(NT), 0, st, B);
+  for (int side = 0; side < 6; side++) {
+    if (ghost[side] != 1 && ghost[side] != 2) continue;
+    const int axis = side / 2;
+    const int64_t cnt = (int64_t)LAP * (axis == 0 ? (int64_t)g.ny * g.nz : (axis == 1 ? (int64_t)g.nx * g.nz : (int64_t)g.nx * g.ny));
+    hipLaunchKernelGGL(k_ghost, dim3(nb(cnt)), dim3(NT), 0, st, B, side);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const Block &B, int axis, int scheme, double s0, double rth, const int rng[6],
+                         double *fout, double *p, hipStream_t st) {
+  (void)p;
+  FilterArgs A;
+  A.axis = axis;
+  A.scheme = scheme;
+  A.s0 = s0;
+  A.rth = rth;
+  for (int q = 0; q < 3; q++) { A.lo[q] = rng[2 * q]; A.hi[q] = rng[2 * q + 1]; }
+  const int n = B.g.n(axis);
+  if (B.S.iperiodic[axis] != 1) {
+    const int lo_min = scheme == 1 ? 6 : 4, hi_off = scheme == 1 ? 5 : 3;
+    if (A.lo[axis] < lo_min) A.lo[axis] = lo_min;
+    if (A.hi[axis] > n - hi_off) A.hi[axis] = n - hi_off;
+  }
+  hipLaunchKernelGGL(k_filter, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, A, fout);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_flow_msg(const Block &B, double *partial, double *out, hipStream_t st) {
+  const unsigned nblk = nb(B.g.ni);
+  hipLaunchKernelGGL(k_flow_msg, dim3(nblk), dim3(NT), 0, st, B, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(NT), 0, st, (const double *)partial, (int64_t)nblk, 3, 3, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ana_box(const Block &B, double *grad, double *amu, double *partial, double *out, hipStream_t st) {
+  (void)grad;
+  (void)amu;
+  const unsigned nblk = nb(B.g.ni);
+  hipLaunchKernelGGL(k_ana_box, dim3(nblk), dim3(NT), 0, st, B, partial, (const double *)nullptr);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(NT), 0, st, (const double *)partial, (int64_t)nblk, 14, 14, out);
+  // pass 2: sum (d - <d>)^2 with <d> = out[6] / N
+  hipLaunchKernelGGL(k_ana_box, dim3(nblk), dim3(NT), 0, st, B, partial, (const double *)(out + 6));
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(NT), 0, st, (const double *)partial, (int64_t)nblk, 1, 1, out + 14);
+  return hipGetLastError();
+}
+
+hipError_t launch_metrics(const Block &B, double *coords[3], double *met[M_N], const double spans[9],
+                          const int ghost[6], double *grad, hipStream_t st) {
+  (void)grad;
+  const Geom &g = B.g;
+  // halos of the coordinates, then the periodic shift
+  (void)launch_halo_fill(B, coords, 3, st);
+  for (int a = 0; a < 3; a++) {
+    if (B.S.iperiodic[a] != 1) continue;
+    const int64_t cnt = (int64_t)2 * LAP * (a == 0 ? (int64_t)g.ny * g.nz : (a == 1 ? (int64_t)g.nx * g.nz : (int64_t)g.nx * g.ny));
+    hipLaunchKernelGGL(k_coord_shift, dim3(nb(cnt)), dim3(NT), 0, st, g, coords[0], coords[1], coords[2], a,
+                       spans[3 * a], spans[3 * a + 1], spans[3 * a + 2]);
+  }
+  MetPtrs mp;
+  for (int m = 0; m < M_N; m++) mp.p[m] = met[m];
+  hipLaunchKernelGGL(k_jacobian, dim3(nb(g.ni)), dim3(NT), 0, st, B, (const double *)coords[0],
+                     (const double *)coords[1], (const double *)coords[2], mp);
+  (void)launch_halo_fill(B, met, M_AJAC + 1, st);
+  // Jac_Ghost_Extent_1st for every side first, then _2nd (OCFD_GhostCell.f90:1-7)
+  for (int side = 0; side < 6; side++) {
+    if (ghost[side] != 1) continue;
+    const int axis = side / 2;
+    const int64_t cnt = (int64_t)LAP * (axis == 0 ? (int64_t)g.ny * g.nz : (axis == 1 ? (int64_t)g.nx * g.nz : (int64_t)g.nx * g.ny));
+    hipLaunchKernelGGL(k_jac_ghost1, dim3(nb(cnt)), dim3(NT), 0, st, g, side, coords[0], coords[1], coords[2], mp);
+  }
+  for (int side = 0; side < 6; side++) {
+    if (ghost[side] != 2) continue;
+    const int axis = side / 2;
+    const int64_t cnt = (int64_t)LAP * (axis == 0 ? (int64_t)g.ny * g.nz : (axis == 1 ? (int64_t)g.nx * g.nz : (int64_t)g.nx * g.ny));
+    hipLaunchKernelGGL(k_jac_ghost2_coords, dim3(nb(cnt)), dim3(NT), 0, st, g, side, coords[0], coords[1], coords[2]);
+    hipLaunchKernelGGL(k_jac_ghost2, dim3(nb(cnt)), dim3(NT), 0, st, B, side, (const double *)coords[0],
+                       (const double *)coords[1], (const double *)coords[2], mp);
+  }
+  return launch_normalise_metrics(B, met, st);
+}
+
+hipError_t launch_normalise_metrics(const Block &B, double *met[M_N], hipStream_t st) {
+  MetPtrs mp;
+  for (int m = 0; m < M_N; m++) mp.p[m] = met[m];
+  hipLaunchKernelGGL(k_normalise, dim3(nb(B.g.nh)), dim3(NT), 0, st, B.g, mp);
+  return hipGetLastError();
+}
+
+}  // namespace ocfd

@@ -1,0 +1,61 @@
+"""The C ABI library loads on a machine without a GPU and exports every entry
+point declared in include/ocfd_hip.h (no compute call is made here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from ocfd import native
+from ocfd.config import OcfdConfigC
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ocfd_hip.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ocfd_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_reference_routine_set():
+    names = declared()
+    for n in ("ocfd_create", "ocfd_destroy", "ocfd_du_inviscous", "ocfd_du_inviscous_character",
+              "ocfd_du_viscous", "ocfd_time_adv_rk3", "ocfd_comput_duvwt", "ocfd_bc",
+              "ocfd_comput_rhybrid", "ocfd_filtering", "ocfd_exchange_primitives", "ocfd_step",
+              "ocfd_show_flow_msg", "ocfd_ana_box"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(native.LIB_PATH):
+        pytest.fail("libocfd_hip.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(native.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_struct_layout_matches_header():
+    """Offsets of the ctypes mirror follow the C declaration order (int32
+    block, then doubles)."""
+    c = OcfdConfigC
+    assert c.nx.offset == 0
+    assert c.iperiodic.offset == 16
+    assert c.ma.offset % 8 == 0
+    assert ctypes.sizeof(c) == c.reserved_d.offset + 8 * 8
+    # 36 int32 fields before the doubles: 4+3+7+6+6+4+8 = 38 -> padded to 8 bytes
+    assert c.ma.offset == 152
+
+
+def test_create_without_gpu_fails_cleanly():
+    from util import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    lib = native.load_library()
+    from ocfd.config import CaseConfig
+    cfg = CaseConfig(nx=8, ny=8, nz=8, iperiodic=(1, 1, 1)).to_c()
+    h = ctypes.c_void_p()
+    rc = lib.ocfd_create(ctypes.byref(cfg), 0, ctypes.byref(h))
+    assert rc < 0
+    assert lib.ocfd_last_error()

@@ -1,0 +1,172 @@
+"""HIP path (through the C ABI) against the golden vectors of the compiled
+reference and against the C oracle.
+
+Tolerances (FP64 path, per field):
+  per-routine outputs and whole steps on the golden cases: bitwise, i.e.
+    relative L2 == 0 (the kernels keep the reference's operation order and
+    are built without FMA contraction);
+  100-step TGV and isotropic runs against the oracle: relative L2 <= 1e-12
+    per field, kinetic-energy / enstrophy histories <= 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from ocfd import cases, native
+from util import golden_names, interior, load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NAMES = golden_names()
+
+
+def gpu_block(case, uniform=None):
+    return native.setup_block(native.Block(case.cfg), case, use_uniform=uniform)
+
+
+def assert_same(got, want, what):
+    err = rel_l2(got, want)
+    nbad = int(np.count_nonzero(got != want))
+    assert nbad == 0, "%s: %d values differ, relative L2 %.3e" % (what, nbad, err)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_per_routine_bitwise(name):
+    case, ref = load_golden(name)
+    cfg = case.cfg
+    with gpu_block(case) as b:
+        if "met" in ref:
+            mets = b.download_metrics()
+            for q, m in enumerate((9, 10, 14, 18)):
+                assert_same(mets[m], ref["met"][q], "metric %d" % m)
+        b.save_fn()
+        b.exchange_primitives()
+        if cfg.scheme_invis.upper() == "HYBRID":
+            b.comput_rhybrid()
+            assert_same(interior(b.download_rhybrid()), ref["rhyb"], "Rhybrid")
+        if cfg.if_scheme_character:
+            b.du_inviscous_character()
+        else:
+            b.du_inviscous()
+        assert_same(b.download_du(), ref["k1_duinv"], "du inviscid")
+        b.du_viscous()
+        assert_same(b.download_du(), ref["k1_dutot"], "du total")
+        b.time_adv_rk3(1)
+        assert_same(b.download_f(), ref["k1_f"], "f after stage 1")
+        b.comput_duvwt()
+        b.bc()
+        for krk in (2, 3):
+            b.stage(krk)
+        b.advance_clock()
+        b.filtering()
+        assert_same(b.download_f(), ref["n1_f"], "f after one step")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_whole_steps_bitwise(name):
+    """ocfd_step (fused stage kernels) over three steps against the reference."""
+    case, ref = load_golden(name)
+    with gpu_block(case) as b:
+        b.step(1)
+        assert_same(b.download_f(), ref["n1_f"], "f after ocfd_step(1)")
+        b.step(2)
+        assert_same(b.download_f(), ref["nN_f"], "f after ocfd_step(3)")
+        istep, tt = b.get_time()
+        assert istep == 3
+
+
+@pytest.mark.parametrize("name", ["w5_sw", "hyb_c_llf", "curv_wall_fo9p", "blunt_massforce"])
+def test_golden_diagnostics(name):
+    case, ref = load_golden(name)
+    with gpu_block(case) as b:
+        for row in ref["hist"]:
+            b.step(1)
+            e = b.show_flow_msg()
+            a = b.ana_box()
+            np.testing.assert_allclose(e, row[2:5], rtol=1e-13, atol=0)
+            np.testing.assert_allclose([a[12], a[13]], row[5:7], rtol=1e-12, atol=0)
+
+
+MID = [
+    dict(seed=31, scheme="WENO5", split=1, char=0),
+    dict(seed=32, scheme="WENO7", split=2, char=1),
+    dict(seed=33, scheme="HYBRID", split=2, char=1, jump=0.5, hybrid=(0.002, 0.02)),
+    dict(seed=34, scheme="OMP6", split=1, char=1, noise=0.03),
+    dict(seed=35, scheme="WENO5", split=1, char=1, grid="curv", iperiodic=(1, 0, 1),
+         scheme_boundary=(0, 0, 1, 0, 0, 0), ghost_cell=(0, 0, 2, 0, 0, 0), bc="BC_USER_DEF",
+         bc_para={1: 1.05}, filters=[[1, 1, 1, 1, 1, 40, 1, 40, 1, 40, 1, 0.2, 0.0]]),
+]
+
+
+@pytest.mark.parametrize("kw", MID, ids=[m["scheme"] + str(m["seed"]) for m in MID])
+def test_oracle_mid_size_routines_and_steps(kw):
+    """Non-cubic 40x28x20 blocks (several tiles per axis, partial tiles)."""
+    case = cases.random_case(shape=(20, 28, 40), **kw)
+    ob = pyoracle.setup_oracle(case)
+    with gpu_block(case) as b:
+        for blk in (ob, b):
+            blk.save_fn()
+            blk.exchange_primitives()
+            if case.cfg.scheme_invis == "HYBRID":
+                blk.comput_rhybrid()
+            (blk.du_inviscous_character if case.cfg.if_scheme_character else blk.du_inviscous)()
+        assert_same(b.download_du(), ob.download_du(), "du inviscid")
+        ob.step(3)
+        b.step(3)
+        assert_same(b.download_f(), ob.download_f(), "f after 3 steps")
+        pg, po = b.download_primitives(), ob.download_primitives()
+        for q in range(5):
+            assert_same(interior(pg[q]), interior(po[q]), "primitive %d" % q)
+
+
+def test_fused_step_equals_routine_sequence():
+    case = cases.random_case(shape=(16, 20, 24), seed=41, scheme="WENO7", split=2, char=1)
+    with gpu_block(case) as a, gpu_block(case) as b:
+        a.step(2)
+        b.step_by_routines(2)
+        assert_same(a.download_f(), b.download_f(), "fused vs per-routine f")
+
+
+def test_uniform_metrics_mode_matches_oracle():
+    case = cases.isotropic(32, scheme="WENO5", split=1, char=0, metric_mode=1)
+    ob = pyoracle.setup_oracle(case, uniform=True)
+    with gpu_block(case, uniform=True) as b:
+        ob.step(3)
+        b.step(3)
+        assert_same(b.download_f(), ob.download_f(), "uniform-metric f after 3 steps")
+
+
+def test_negative_temperature_trap_and_repair():
+    """Three isolated cells get negative energy: comput_duvwT reports them and
+    replaces T by the mean of the six neighbours (OCFD_handle_NegativeT.f90)."""
+    case = cases.random_case(shape=(12, 14, 16), seed=43)
+    ob = pyoracle.setup_oracle(case)
+    with gpu_block(case) as b:
+        f = ob.download_f()
+        for (k, j, i) in ((2, 3, 4), (7, 9, 11), (10, 2, 14)):
+            f[4, k, j, i] = -1.0
+        ob.upload_f(f)
+        b.upload_f(f)
+        assert ob.comput_duvwt() == 3
+        assert b.comput_duvwt() == 3
+        for q in range(5):
+            assert_same(interior(b.download_primitives()[q]), interior(ob.download_primitives()[q]), "prim %d" % q)
+
+
+@pytest.mark.slow
+def test_tgv64_100_steps_against_oracle():
+    """Config-1 TGV, 64^3, WENO5 + Steger-Warming, CD6, CFL 0.5, 100 steps."""
+    case = cases.tgv(64)
+    ob = pyoracle.setup_oracle(case)
+    with gpu_block(case) as b:
+        ke_o, ke_g, en_o, en_g = [], [], [], []
+        for _ in range(10):
+            ob.step(10)
+            b.step(10)
+            ao, ag = ob.ana_box(), b.ana_box()
+            ke_o.append(ao[12]); ke_g.append(ag[12]); en_o.append(ao[13]); en_g.append(ag[13])
+        pg, po = b.download_primitives(), ob.download_primitives()
+        for q, nm in enumerate("duvwT"):
+            assert rel_l2(interior(pg[q]), interior(po[q])) <= 1e-12, nm
+        np.testing.assert_allclose(ke_g, ke_o, rtol=1e-10)
+        np.testing.assert_allclose(en_g, en_o, rtol=1e-10)
