@@ -656,7 +656,7 @@ static void sound_speed(oracle_block *b) {
 /* Mass force (OCFD_inviscous.f90:189-201) */
 static void mass_force(oracle_block *b) {
     if (b->c.if_mass_force != 1) return;
-    const double *g = b->c.mass_force;
+#define g b->c.mass_force
     for (int k = 1; k <= b->nz; k++) for (int j = 1; j <= b->ny; j++) for (int i = 1; i <= b->nx; i++) {
         size_t q = HI(b, i, j, k);
         b->du[FI(b, i, j, k, 2)] = b->du[FI(b, i, j, k, 2)] + b->d[q] * g[0];
@@ -665,6 +665,7 @@ static void mass_force(oracle_block *b) {
         b->du[FI(b, i, j, k, 5)] = b->du[FI(b, i, j, k, 5)] +
             b->d[q] * (b->u[q] * g[0] + b->v[q] * g[1] + b->w[q] * g[2]);
     }
+#undef g
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1178,8 +1179,8 @@ int oracle_comput_rhybrid(oracle_block *b, int64_t counts[4]) {
     /* forced-WENO patches (OCFD_SetHybrid.f90:293-315), with the j range read
      * from Hybrid_para(7:8) as the comments there intend */
     int mpatch = (int)lround(b->c.hybrid_para[3]);
+#define hp b->c.hybrid_para
     for (int m = 1; m <= mpatch; m++) {
-        const double *hp = b->c.hybrid_para;
         int ib = (int)lround(hp[4 + (m - 1) * 6]), ie = (int)lround(hp[5 + (m - 1) * 6]);
         int jb = (int)lround(hp[6 + (m - 1) * 6]), je = (int)lround(hp[7 + (m - 1) * 6]);
         int kb = (int)lround(hp[8 + (m - 1) * 6]), ke = (int)lround(hp[9 + (m - 1) * 6]);
@@ -1192,6 +1193,7 @@ int oracle_comput_rhybrid(oracle_block *b, int64_t counts[4]) {
         for (int k = kb; k <= ke; k++) for (int j = jb; j <= je; j++) for (int i = ib; i <= ie; i++)
             b->rhyb[HI(b, i, j, k)] = b->rhyb[HI(b, i, j, k)] + 100.0;
     }
+#undef hp
     exchange_xyz(b, b->rhyb);
     if (counts) {
         int64_t ns[3] = {0, 0, 0};
@@ -1296,8 +1298,8 @@ int oracle_filtering(oracle_block *b) {
         if (per != 0 && b->istep % per == 0) any = 1;
     }
     if (!any) return 0;
+#define fp b->c.filter_para[m]
     for (int m = 0; m < nf; m++) {
-        const double *fp = b->c.filter_para[m];
         int per = (int)lround(fp[0]);
         if (per == 0 || b->istep % per != 0) continue;
         int fx = (int)lround(fp[1]), fy = (int)lround(fp[2]), fz = (int)lround(fp[3]);
@@ -1318,6 +1320,7 @@ int oracle_filtering(oracle_block *b) {
             else if (scheme == 2) filter_shock(b, axis, s0, rth, rng);
         }
     }
+#undef fp
     return 0;
 }
 

@@ -19,12 +19,26 @@ from ocfd.config import CaseConfig, OcfdConfigC  # noqa: E402
 from ocfd.native import BlockBase, OcfdError, _ptr  # noqa: E402
 
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+COUNT_LIB_PATH = os.path.join(HERE, "liboracle_count.so")
 REF_DIR = os.path.join(HERE, "_ref")
 PROBE = os.path.join(REF_DIR, "ocfd_probe")
 REF_EXE = os.path.join(REF_DIR, "opencfd_ref")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 _lib = None
+_count_lib = None
+
+
+def load_counting():
+    """The operation-counting build of the oracle (oracle_count.cpp)."""
+    global _count_lib
+    if _count_lib is None:
+        if not os.path.exists(COUNT_LIB_PATH):
+            raise OcfdError("counting oracle not built: %s" % COUNT_LIB_PATH)
+        lib = ctypes.CDLL(COUNT_LIB_PATH)
+        lib.oracle_create.argtypes = [ctypes.POINTER(OcfdConfigC), ctypes.POINTER(ctypes.c_void_p)]
+        _count_lib = lib
+    return _count_lib
 
 
 def load():
@@ -49,8 +63,8 @@ def load():
 class OracleBlock(BlockBase):
     prefix = "oracle"
 
-    def __init__(self, cfg: CaseConfig):
-        super().__init__(load(), cfg)
+    def __init__(self, cfg: CaseConfig, counting: bool = False):
+        super().__init__(load_counting() if counting else load(), cfg)
         h = ctypes.c_void_p()
         if self.lib.oracle_create(ctypes.byref(self._c), ctypes.byref(h)) != 0:
             raise OcfdError("oracle_create failed")
@@ -84,12 +98,12 @@ class OracleBlock(BlockBase):
             pass
 
 
-def setup_oracle(case, uniform=None):
+def setup_oracle(case, uniform=None, counting=False):
     """Oracle block in the state the reference reaches after init3d + OCFD_bc."""
     cfg = case.cfg
     if uniform is None:
         uniform = cfg.metric_mode == 1
-    b = OracleBlock(cfg)
+    b = OracleBlock(cfg, counting=counting)
     if uniform:
         b.set_grid_and_jacobian(case.x, case.y, case.z, case.spans, case.uniform_metrics)
     else:
@@ -174,3 +188,14 @@ def run_reference(case, workdir, nsteps, nranks=1, decomposition=(1, 1, 1), time
     if os.path.exists(fn):
         _, _, prims = fortran_io.read_flow_data(fn, case.cfg.shape)
     return wall, prims, r.stdout
+
+
+def count_ops(block, fn):
+    """Floating-point operations (add/sub, mul, div, sqrt, pow) executed by
+    fn(block) on a counting oracle block."""
+    lib = block.lib
+    lib.oracle_count_reset()
+    fn(block)
+    out = (ctypes.c_int64 * 5)()
+    lib.oracle_count_read(out)
+    return list(out)
