@@ -35,7 +35,8 @@ from ocfd import cases, native, perfmodel  # noqa: E402
 from ocfd.config import METRIC_FIELD, METRIC_UNIFORM  # noqa: E402
 
 METRIC = "Mcell-RK3-steps/s, FP64 vs mixed FP32/FP64 at stated L2 error; HBM GB/s"
-HYB_THRESHOLDS = (0.002, 0.02)   # Hybrid_para(2:3): UD7L below, WENO7 between, WENO5 above
+HYB_THRESHOLDS = (0.01, 0.05)    # Hybrid_para(2:3): UD7L below, WENO7 between, WENO5 above
+SPINUP_STEPS = 100               # untimed steps that turn the seeded field into developed turbulence
 
 
 def log(*a):
@@ -123,6 +124,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--spinup", type=int, default=-1,
+                    help="untimed steps before warm-up that develop the initial field (default %d for the "
+                         "isotropic workloads, 0 otherwise)" % SPINUP_STEPS)
     ap.add_argument("--workload", default="iso256")
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
@@ -142,6 +146,13 @@ def main():
     cells = cfg.nx * cfg.ny * cfg.nz
     log("case %s built in %.1f s (dt=%.4g)" % (case.name, time.time() - t0, cfg.dt))
     b = native.setup_block(native.Block(cfg), case)
+    spinup = args.spinup if args.spinup >= 0 else (SPINUP_STEPS if args.workload.startswith("iso") else 0)
+    if spinup:
+        t1 = time.time()
+        b.step(spinup)
+        b.synchronize()
+        log("spin-up: %d steps in %.1f s" % (spinup, time.time() - t1))
+        desc["spinup_steps"] = spinup
     b.step(args.warmup)
     b.synchronize()
     mix0 = hybrid_mix(b) if cfg.scheme_invis == "HYBRID" else None

@@ -525,35 +525,36 @@ __device__ inline void char_back(const CharBasis &B, const double *h0, double *h
 }
 
 // ---------------------------------------------------------------------------
-// central first derivative of a field along `axis` at interior point s
-// (OCFD_dx0/dy0/dz0, OCFD_dxyz_viscous.f90:3-239).  `at(t)` returns the
-// field at position t on the line (t may lie in the halo for periodic axes).
+// central first derivative of a field along an axis at interior point s of n
+// (OCFD_dx0/dy0/dz0, OCFD_dxyz_viscous.f90:3-239).  `o(d)` returns the field
+// at offset d (|d| <= 4) from s; every branch uses constant offsets, so a
+// caller can serve o() from registers.  Block-edge closures apply on
+// non-periodic axes only.
 // ---------------------------------------------------------------------------
 template <typename F>
-__device__ inline double dxyz0_point(F at, int s, int n, double h, bool cd8, bool periodic) {
+__device__ inline double dxyz0_point(F o, int s, int n, double h, bool cd8, bool periodic) {
   if (!periodic) {
-    if (s == 1) return (-(3.0 * at(1)) + 4.0 * at(2) - at(3)) / (2.0 * h);
-    if (s == 2) return (at(3) - at(1)) / (2.0 * h);
-    if (s == 3) return (8.0 * (at(4) - at(2)) - (at(5) - at(1))) / (12.0 * h);
+    if (s == 1) return (-(3.0 * o(0)) + 4.0 * o(1) - o(2)) / (2.0 * h);
+    if (s == 2) return (o(1) - o(-1)) / (2.0 * h);
+    if (s == 3) return (8.0 * (o(1) - o(-1)) - (o(2) - o(-2))) / (12.0 * h);
     if (cd8 && s == 4) {
       const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-      return a1 * (at(7) - at(1)) + a2 * (at(6) - at(2)) + a3 * (at(5) - at(3));
+      return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
     }
-    if (s == n) return (at(n - 2) - 4.0 * at(n - 1) + 3.0 * at(n)) / (2.0 * h);
-    if (s == n - 1) return (at(n) - at(n - 2)) / (2.0 * h);
-    if (s == n - 2) return (8.0 * (at(n - 1) - at(n - 3)) - (at(n) - at(n - 4))) / (12.0 * h);
+    if (s == n) return (o(-2) - 4.0 * o(-1) + 3.0 * o(0)) / (2.0 * h);
+    if (s == n - 1) return (o(1) - o(-1)) / (2.0 * h);
+    if (s == n - 2) return (8.0 * (o(1) - o(-1)) - (o(2) - o(-2))) / (12.0 * h);
     if (cd8 && s == n - 3) {
       const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-      return a1 * (at(n) - at(n - 6)) + a2 * (at(n - 1) - at(n - 5)) + a3 * (at(n - 2) - at(n - 4));
+      return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
     }
   }
   if (!cd8) {
     const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-    return a1 * (at(s + 3) - at(s - 3)) + a2 * (at(s + 2) - at(s - 2)) + a3 * (at(s + 1) - at(s - 1));
+    return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
   }
   const double c1 = 0.8 / h, c2 = -(0.2 / h), c3 = 3.80952380952380952e-2 / h, c4 = -(3.571428571428571428e-3 / h);
-  return c1 * (at(s + 1) - at(s - 1)) + c2 * (at(s + 2) - at(s - 2)) + c3 * (at(s + 3) - at(s - 3)) +
-         c4 * (at(s + 4) - at(s - 4));
+  return c1 * (o(1) - o(-1)) + c2 * (o(2) - o(-2)) + c3 * (o(3) - o(-3)) + c4 * (o(4) - o(-4));
 }
 
 }  // namespace ocfd

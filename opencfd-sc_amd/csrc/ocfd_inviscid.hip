@@ -50,6 +50,11 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   double *sQ = sF + 10 * NP * TW;          // [7][NP][TW]  (characteristic only)
   double *sR = sQ + (CHAR ? 7 * NP * TW : 0);  // [NP][TW] (hybrid only)
   double *sH = sR + (HYB ? NP * TW : 0);   // [NH][NF][TW]
+  // hybrid scheme: WENO7 faces listed from the front, WENO5 from the back
+  constexpr int NITEM = CHAR ? NF * TW : 5 * NF * TW;
+  int *sL = reinterpret_cast<int *>(sH + NH * NF * TW);  // [NITEM] (hybrid only)
+  __shared__ int s_cnt[2];
+  if (HYB && threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
 
   const Geom &g = B.g;
   const int n = g.n(AXIS);
@@ -97,6 +102,78 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
 
   // ---- phase 2: face fluxes ----
   const SchemeParams &S = B.S;
+  const int lo = S.bidx[AXIS][0], hi = S.bidx[AXIS][1];
+  const int sl = S.sb[2 * AXIS], sr = S.sb[2 * AXIS + 1];
+  // flux+/- of component m at face f (non-characteristic)
+  auto face_nc = [&](int m, int f, int wl, int s, int scm) {
+    double xp[8], xm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      xp[k] = sF[(m * NP + f + k) * TW + wl];
+      xm[k] = sF[((5 + m) * NP + f + k) * TW + wl];
+    }
+    sH[(m * NF + f) * TW + wl] = face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
+    sH[((5 + m) * NF + f) * TW + wl] = face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
+  };
+#define FPV(m, k) sF[((m) * NP + f + (k)) * TW + wl]
+#define FMV(m, k) sF[((5 + (m)) * NP + f + (k)) * TW + wl]
+  // all five components of face f, projected on the characteristic basis of
+  // the face unless the face is linear (flux_charteric,
+  // OCFD_inviscous_character.f90:190-290)
+  auto face_ch = [&](int f, int wl, int s, int scm) {
+    double hh[5];
+    int a1h = S.ka1, b1h = S.kb1, a2h = S.ka2, b2h = S.kb2;
+    if (HYB) {
+      if (scm == 2) { a1h = -3; b1h = 3; a2h = -2; b2h = 4; }
+      else if (scm == 3) { a1h = -2; b1h = 2; a2h = -1; b2h = 3; }
+    }
+    CharBasis Bs;
+    if (scm != 1) {
+#define QV(q, k) sQ[((q) * NP + f + (k)) * TW + wl]
+      const double u1 = (QV(0, 3) + QV(0, 4)) * 0.5, v1 = (QV(1, 3) + QV(1, 4)) * 0.5;
+      const double w1 = (QV(2, 3) + QV(2, 4)) * 0.5, c1 = (QV(3, 3) + QV(3, 4)) * 0.5;
+      const double a1 = (QV(4, 3) + QV(4, 4)) * 0.5, a2 = (QV(5, 3) + QV(5, 4)) * 0.5;
+      const double a3 = (QV(6, 3) + QV(6, 4)) * 0.5;
+#undef QV
+      char_basis(gamma, u1, v1, w1, c1, a1, a2, a3, Bs);
+    }
+    double h0[5];
+    for (int r = 0; r < 5; r++) {
+      double xp[8], xm[8];
+      if (scm == 1) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) { xp[k] = FPV(r, k); xm[k] = FMV(r, k); }
+      } else {
+        double sr5[5];
+        char_S_row(Bs, r, sr5);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          xp[k] = 0.0;
+          xm[k] = 0.0;
+          if (k >= a1h + 3 && k <= b1h + 3)
+            xp[k] = sr5[0] * FPV(0, k) + sr5[1] * FPV(1, k) + sr5[2] * FPV(2, k) + sr5[3] * FPV(3, k) +
+                    sr5[4] * FPV(4, k);
+          if (k >= a2h + 3 && k <= b2h + 3)
+            xm[k] = sr5[0] * FMV(0, k) + sr5[1] * FMV(1, k) + sr5[2] * FMV(2, k) + sr5[3] * FMV(3, k) +
+                    sr5[4] * FMV(4, k);
+        }
+      }
+      const double h1 = face_p_char<SCH>(S, xp, s, n, AXIS, scm);
+      const double h2 = face_m_char<SCH>(S, xm, s, n, AXIS, scm);
+      h0[r] = h1 + h2;
+    }
+    if (scm == 1) {
+      for (int r = 0; r < 5; r++) hh[r] = h0[r];
+    } else {
+      char_back(Bs, h0, hh);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) sH[(m * NF + f) * TW + wl] = hh[m];
+  };
+  auto push = [&](int it, int scm) {
+    if (scm == 2) sL[atomicAdd(&s_cnt[0], 1)] = it;
+    else sL[NITEM - 1 - atomicAdd(&s_cnt[1], 1)] = it;
+  };
   if (!CHAR) {
     for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
       const int wl = it % TW;
@@ -106,93 +183,54 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
       if (s > n || w0 + wl > nw) continue;
       int scm = 0;
       if (HYB) scm = hybrid_index(dmax2(sR[(f + 3) * TW + wl], sR[(f + 4) * TW + wl]), S.hyb2, S.hyb3);
-      double xp[8], xm[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        xp[k] = sF[(m * NP + f + k) * TW + wl];
-        xm[k] = sF[((5 + m) * NP + f + k) * TW + wl];
-      }
-      sH[(m * NF + f) * TW + wl] = face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
-      sH[((5 + m) * NF + f) * TW + wl] = face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
+      if (!HYB || scm == 1) face_nc(m, f, wl, s, scm);
+      else push(it, scm);
     }
   } else {
-    const int lo = S.bidx[AXIS][0], hi = S.bidx[AXIS][1];
-    const int sl = S.sb[2 * AXIS], sr = S.sb[2 * AXIS + 1];
     for (int it = tid; it < NF * TW; it += NTHREADS) {
       const int wl = it % TW;
       const int f = it / TW;
       const int s = s0 - 1 + f;
       if (s > n || w0 + wl > nw) continue;
-      double hh[5];
-#define FPV(m, k) sF[((m) * NP + f + (k)) * TW + wl]
-#define FMV(m, k) sF[((5 + (m)) * NP + f + (k)) * TW + wl]
       if ((lo == 1 && sl == 0 && s == 0) || (hi == 1 && sr == 0 && s == n)) {
         // non-reflecting closure (OCFD_inviscous_character.f90:295-309); point t is k = t - s + 3
-        if (s == 0) {
-          for (int m = 0; m < 5; m++)
-            hh[m] = (2.0 * FPV(m, 4) + 5.0 * FPV(m, 5) - FPV(m, 6) + 2.0 * FMV(m, 6) - 7.0 * FMV(m, 5) +
-                     11.0 * FMV(m, 4)) / 6.0;
-        } else {
-          for (int m = 0; m < 5; m++)
-            hh[m] = (2.0 * FPV(m, 1) - 7.0 * FPV(m, 2) + 11.0 * FPV(m, 3) + 2.0 * FMV(m, 3) + 5.0 * FMV(m, 2) -
-                     FMV(m, 1)) / 6.0;
+        for (int m = 0; m < 5; m++) {
+          double hv;
+          if (s == 0)
+            hv = (2.0 * FPV(m, 4) + 5.0 * FPV(m, 5) - FPV(m, 6) + 2.0 * FMV(m, 6) - 7.0 * FMV(m, 5) +
+                  11.0 * FMV(m, 4)) / 6.0;
+          else
+            hv = (2.0 * FPV(m, 1) - 7.0 * FPV(m, 2) + 11.0 * FPV(m, 3) + 2.0 * FMV(m, 3) + 5.0 * FMV(m, 2) -
+                  FMV(m, 1)) / 6.0;
+          sH[(m * NF + f) * TW + wl] = hv;
         }
-      } else {
-        int scm = 0;
-        if (HYB) scm = hybrid_index(dmax2(sR[(f + 3) * TW + wl], sR[(f + 4) * TW + wl]), S.hyb2, S.hyb3);
-        // projected stencil ranges (flux_charteric, OCFD_inviscous_character.f90:196-202)
-        int a1h = S.ka1, b1h = S.kb1, a2h = S.ka2, b2h = S.kb2;
-        if (HYB) {
-          if (scm == 2) { a1h = -3; b1h = 3; a2h = -2; b2h = 4; }
-          else if (scm == 3) { a1h = -2; b1h = 2; a2h = -1; b2h = 3; }
-        }
-        CharBasis Bs;
-        if (scm != 1) {
-#define QV(q, k) sQ[((q) * NP + f + (k)) * TW + wl]
-          const double u1 = (QV(0, 3) + QV(0, 4)) * 0.5, v1 = (QV(1, 3) + QV(1, 4)) * 0.5;
-          const double w1 = (QV(2, 3) + QV(2, 4)) * 0.5, c1 = (QV(3, 3) + QV(3, 4)) * 0.5;
-          const double a1 = (QV(4, 3) + QV(4, 4)) * 0.5, a2 = (QV(5, 3) + QV(5, 4)) * 0.5;
-          const double a3 = (QV(6, 3) + QV(6, 4)) * 0.5;
-#undef QV
-          char_basis(gamma, u1, v1, w1, c1, a1, a2, a3, Bs);
-        }
-        double h0[5];
-        for (int r = 0; r < 5; r++) {
-          double xp[8], xm[8];
-          if (scm == 1) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) { xp[k] = FPV(r, k); xm[k] = FMV(r, k); }
-          } else {
-            double sr5[5];
-            char_S_row(Bs, r, sr5);
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-              xp[k] = 0.0;
-              xm[k] = 0.0;
-              if (k >= a1h + 3 && k <= b1h + 3)
-                xp[k] = sr5[0] * FPV(0, k) + sr5[1] * FPV(1, k) + sr5[2] * FPV(2, k) + sr5[3] * FPV(3, k) +
-                        sr5[4] * FPV(4, k);
-              if (k >= a2h + 3 && k <= b2h + 3)
-                xm[k] = sr5[0] * FMV(0, k) + sr5[1] * FMV(1, k) + sr5[2] * FMV(2, k) + sr5[3] * FMV(3, k) +
-                        sr5[4] * FMV(4, k);
-            }
-          }
-          const double h1 = face_p_char<SCH>(S, xp, s, n, AXIS, scm);
-          const double h2 = face_m_char<SCH>(S, xm, s, n, AXIS, scm);
-          h0[r] = h1 + h2;
-        }
-        if (scm == 1) {
-          for (int r = 0; r < 5; r++) hh[r] = h0[r];
-        } else {
-          char_back(Bs, h0, hh);
-        }
+        continue;
       }
-#undef FPV
-#undef FMV
-#pragma unroll
-      for (int m = 0; m < 5; m++) sH[(m * NF + f) * TW + wl] = hh[m];
+      int scm = 0;
+      if (HYB) scm = hybrid_index(dmax2(sR[(f + 3) * TW + wl], sR[(f + 4) * TW + wl]), S.hyb2, S.hyb3);
+      if (!HYB || scm == 1) face_ch(f, wl, s, scm);
+      else push(it, scm);
     }
   }
+  if (HYB) {
+    // second pass: the listed WENO7 faces, then the WENO5 faces, spread over
+    // all threads
+    __syncthreads();
+    const int n7 = s_cnt[0], n5 = s_cnt[1];
+    for (int q = tid; q < n7 + n5; q += NTHREADS) {
+      const int it = q < n7 ? sL[q] : sL[NITEM - 1 - (q - n7)];
+      const int scm = q < n7 ? 2 : 3;
+      if (!CHAR) {
+        const int wl = it % TW, f = (it / TW) % NF, m = it / (TW * NF);
+        face_nc(m, f, wl, s0 - 1 + f, scm);
+      } else {
+        const int wl = it % TW, f = it / TW;
+        face_ch(f, wl, s0 - 1 + f, scm);
+      }
+    }
+  }
+#undef FPV
+#undef FMV
   __syncthreads();
 
   // ---- phase 3: flux difference into du ----
@@ -235,7 +273,8 @@ hipError_t launch_one(const Block &B, int mode, hipStream_t st) {
   constexpr int NP = TL + 8, NF = TL + 1;
   constexpr bool HYB = (SCH == SCH_HYBRID);
   constexpr size_t lds = sizeof(double) * (10 * NP * TW + (CHAR ? 7 * NP * TW : 0) + (HYB ? NP * TW : 0) +
-                                           (CHAR ? 5 : 10) * NF * TW);
+                                           (CHAR ? 5 : 10) * NF * TW) +
+                         (HYB ? sizeof(int) * (CHAR ? NF * TW : 5 * NF * TW) : 0);
   static_assert(lds <= 64 * 1024, "tile too large for LDS");
   const Geom &g = B.g;
   const int n = g.n(AXIS);
