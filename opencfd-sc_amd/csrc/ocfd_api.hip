@@ -212,7 +212,7 @@ int inviscid(ocfd_handle *h, bool ch) {
                                  {"inviscid_char_x", "inviscid_char_y", "inviscid_char_z"}};
   const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
   for (int a = 0; a < 3; a++)
-    if (launch(h, nm[ch][a], [&] { return launch_inviscid(B, a, modes[a], SweepTile{0, 0}, h->st); })) return -1;
+    if (launch(h, nm[ch][a], [&] { return launch_inviscid(B, a, modes[a], SweepTile{0}, h->st); })) return -1;
   return 0;
 }
 

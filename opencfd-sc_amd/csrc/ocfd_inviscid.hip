@@ -38,6 +38,15 @@ struct TileIndex {
   __device__ static inline int nthird(const Geom &g) { return AXIS == 2 ? g.ny : g.nz; }
 };
 
+// LDS element (component m, point or face p, line wl) of a tile array with NPT
+// entries per line: x tiles keep the points of one line contiguous, y and z
+// tiles the lines of one point, so that consecutive lanes touch consecutive
+// doubles in every phase.
+template <int AXIS, int NPT, int TW>
+__device__ constexpr inline int lidx(int m, int p, int wl) {
+  return AXIS == 0 ? (m * TW + wl) * NPT + p : (m * NPT + p) * TW + wl;
+}
+
 template <int AXIS, int SCH, bool CHAR, int TL, int TW>
 __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int mode) {
   typedef TileIndex<AXIS> TI;
@@ -45,20 +54,20 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   constexpr int NF = TL + 1;  // faces  s0-1 .. s0+TL-1
   constexpr bool HYB = (SCH == SCH_HYBRID);
   constexpr int NH = CHAR ? 5 : 10;
+  constexpr int NI1 = (NP * TW + NTHREADS - 1) / NTHREADS;      // points per thread
+  constexpr int NI3 = (5 * TL * TW + NTHREADS - 1) / NTHREADS;  // du entries per thread
+  // characteristic inputs per point: u, v, w, c (+ the three normalised
+  // metrics when they vary in space)
+  const bool unif = B.M.uniform != 0;
+  const int nq = unif ? 4 : 7;
   extern __shared__ double smem[];
-  double *sF = smem;                       // [10][NP][TW]
-  double *sQ = sF + 10 * NP * TW;          // [7][NP][TW]  (characteristic only)
-  double *sR = sQ + (CHAR ? 7 * NP * TW : 0);  // [NP][TW] (hybrid only)
-  double *sH = sR + (HYB ? NP * TW : 0);   // [NH][NF][TW]
   // hybrid scheme: WENO7 faces listed from the front, WENO5 from the back
-  constexpr int NITEM = CHAR ? NF * TW : 5 * NF * TW;
-  int *sL = reinterpret_cast<int *>(sH + NH * NF * TW);  // [NITEM] (hybrid only)
+  constexpr int NITEM = CHAR ? NF * TW : 5 * NF * TW;  // list capacity
   __shared__ int s_cnt[2];
-  if (HYB && threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
 
   const Geom &g = B.g;
   const int n = g.n(AXIS);
-  const int nw = TI::nwidth(g);
+  const int nw = TI::nwidth(g), nc = TI::nthird(g);
   const int nts = (n + TL - 1) / TL, ntw = (nw + TW - 1) / TW;
   int b = blockIdx.x;
   const int ts = b % nts; b /= nts;
@@ -68,165 +77,292 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   const int tid = threadIdx.x;
   const double gamma = B.P.gamma, rma = B.P.ma;
   const int q1 = M_AKX1 + 3 * AXIS;
-
-  // ---- phase 1: split fluxes at the tile's points ----
-  for (int it = tid; it < NP * TW; it += NTHREADS) {
-    int p, wl;
-    if (AXIS == 0) { p = it % NP; wl = it / NP; } else { wl = it % TW; p = it / TW; }
-    const int s = s0 - 4 + p, w = w0 + wl;
-    if (s > n + LAP || w > nw) continue;
-    const int64_t h = TI::hidx(g, s, w, c);
-    const double dd = B.d[h], uu = B.u[h], vv = B.v[h], ww = B.w[h];
-    const double cc = sqrt(B.T[h]) / rma;  // OCFD_inviscous.f90:67
-    const double A1 = B.M.get(q1, h), A2 = B.M.get(q1 + 1, h), A3 = B.M.get(q1 + 2, h);
-    double fp[5], fm[5];
-    if (B.S.flux_split == 1) split_sw(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
-    else split_llf(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      sF[(m * NP + p) * TW + wl] = fp[m];
-      sF[((5 + m) * NP + p) * TW + wl] = fm[m];
-    }
-    if (CHAR) {
-      sQ[(0 * NP + p) * TW + wl] = uu;
-      sQ[(1 * NP + p) * TW + wl] = vv;
-      sQ[(2 * NP + p) * TW + wl] = ww;
-      sQ[(3 * NP + p) * TW + wl] = cc;
-      sQ[(4 * NP + p) * TW + wl] = A1;
-      sQ[(5 * NP + p) * TW + wl] = A2;
-      sQ[(6 * NP + p) * TW + wl] = A3;
-    }
-    if (HYB) sR[p * TW + wl] = B.rhyb[h];
-  }
-  __syncthreads();
-
-  // ---- phase 2: face fluxes ----
   const SchemeParams &S = B.S;
   const int lo = S.bidx[AXIS][0], hi = S.bidx[AXIS][1];
   const int sl = S.sb[2 * AXIS], sr = S.sb[2 * AXIS + 1];
+
+  // point handled by phase-1 slot r of this thread
+  auto p1 = [&](int r, int &p, int &wl) {
+    const int it = tid + r * NTHREADS;
+    if (AXIS == 0) { p = it % NP; wl = it / NP; } else { wl = it % TW; p = it / TW; }
+    return it < NP * TW && s0 - 4 + p <= n + LAP && w0 + wl <= nw;
+  };
+  // du entry handled by phase-3 slot r
+  auto p3 = [&](int r, int &sl_, int &wl, int &m) {
+    const int it = tid + r * NTHREADS;
+    if (AXIS == 0) { sl_ = it % TL; wl = (it / TL) % TW; m = it / (TL * TW); }
+    else { wl = it % TW; sl_ = (it / TW) % TL; m = it / (TL * TW); }
+    return it < 5 * TL * TW && s0 + sl_ <= n && w0 + wl <= nw;
+  };
+
+  // primitives (+ sensor) of the tile's points
+  double pq[NI1][5], pr[NI1];
+  {
+#pragma unroll
+    for (int r = 0; r < NI1; r++) {
+      int p, wl;
+      if (!p1(r, p, wl)) continue;
+      const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
+      pq[r][0] = B.d[h];
+      pq[r][1] = B.u[h];
+      pq[r][2] = B.v[h];
+      pq[r][3] = B.w[h];
+      pq[r][4] = B.T[h];
+      if (HYB) pr[r] = B.rhyb[h];
+    }
+  }
+  // du entries of the tile (accumulating modes), fetched before the face
+  // fluxes are computed
+  double pdu[NI3];
+
+  double *sF = smem;                               // fp(5), fm(5)      [10][NP][TW]
+  double *sQ = sF + 10 * NP * TW;                  // char inputs       [nq][NP][TW]
+  double *sR = sQ + (CHAR ? nq * NP * TW : 0);     // hybrid sensor     [NP][TW]
+  double *sH = sR + (HYB ? NP * TW : 0);           // face fluxes       [NH][NF][TW]
+  int *sL = reinterpret_cast<int *>(sH + NH * NF * TW);  // [NITEM] (hybrid only)
+  if (HYB && tid < 2) s_cnt[tid] = 0;
+  // ---- phase 1: split fluxes at the tile's points ----
+#pragma unroll
+  for (int r = 0; r < NI1; r++) {
+    int p, wl;
+    if (!p1(r, p, wl)) continue;
+    const double dd = pq[r][0], uu = pq[r][1], vv = pq[r][2], ww = pq[r][3];
+    const double cc = sqrt(pq[r][4]) / rma;  // OCFD_inviscous.f90:67
+    double A1, A2, A3;
+    if (unif) {
+      A1 = B.M.u[q1]; A2 = B.M.u[q1 + 1]; A3 = B.M.u[q1 + 2];
+    } else {
+      const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
+      A1 = B.M.p[q1][h]; A2 = B.M.p[q1 + 1][h]; A3 = B.M.p[q1 + 2][h];
+    }
+    double fp[5], fm[5];
+    if (S.flux_split == 1) split_sw(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+    else split_llf(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      sF[lidx<AXIS, NP, TW>(m, p, wl)] = fp[m];
+      sF[lidx<AXIS, NP, TW>(5 + m, p, wl)] = fm[m];
+    }
+    if (CHAR) {
+      sQ[lidx<AXIS, NP, TW>(0, p, wl)] = uu;
+      sQ[lidx<AXIS, NP, TW>(1, p, wl)] = vv;
+      sQ[lidx<AXIS, NP, TW>(2, p, wl)] = ww;
+      sQ[lidx<AXIS, NP, TW>(3, p, wl)] = cc;
+      if (!unif) {
+        sQ[lidx<AXIS, NP, TW>(4, p, wl)] = A1;
+        sQ[lidx<AXIS, NP, TW>(5, p, wl)] = A2;
+        sQ[lidx<AXIS, NP, TW>(6, p, wl)] = A3;
+      }
+    }
+    if (HYB) sR[lidx<AXIS, NP, TW>(0, p, wl)] = pr[r];
+  }
+  __syncthreads();
+
+  if (mode != INV_ASSIGN) {
+#pragma unroll
+    for (int r = 0; r < NI3; r++) {
+      int sl_, wl, m;
+      if (!p3(r, sl_, wl, m)) continue;
+      pdu[r] = B.du[TI::iidx(g, s0 + sl_, w0 + wl, c) + (int64_t)m * g.ni];
+    }
+  }
+
+  // ---- phase 2: face fluxes ----
   // flux+/- of component m at face f (non-characteristic)
   auto face_nc = [&](int m, int f, int wl, int s, int scm) {
     double xp[8], xm[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      xp[k] = sF[(m * NP + f + k) * TW + wl];
-      xm[k] = sF[((5 + m) * NP + f + k) * TW + wl];
+      xp[k] = sF[lidx<AXIS, NP, TW>(m, f + k, wl)];
+      xm[k] = sF[lidx<AXIS, NP, TW>(5 + m, f + k, wl)];
     }
-    sH[(m * NF + f) * TW + wl] = face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
-    sH[((5 + m) * NF + f) * TW + wl] = face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
+    sH[lidx<AXIS, NF, TW>(m, f, wl)] = face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
+    sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] = face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
   };
-#define FPV(m, k) sF[((m) * NP + f + (k)) * TW + wl]
-#define FMV(m, k) sF[((5 + (m)) * NP + f + (k)) * TW + wl]
+#define FPV(m, k) sF[lidx<AXIS, NP, TW>((m), f + (k), wl)]
+#define FMV(m, k) sF[lidx<AXIS, NP, TW>(5 + (m), f + (k), wl)]
   // all five components of face f, projected on the characteristic basis of
   // the face unless the face is linear (flux_charteric,
   // OCFD_inviscous_character.f90:190-290)
-  auto face_ch = [&](int f, int wl, int s, int scm) {
-    double hh[5];
+  // characteristic basis of face f from the face-averaged state
+  // (OCFD_inviscous_character.f90:200-236)
+  auto basis_of = [&](int f, int wl, CharBasis &Bs) {
+#define QV(q, k) sQ[lidx<AXIS, NP, TW>((q), f + (k), wl)]
+    const double u1 = (QV(0, 3) + QV(0, 4)) * 0.5, v1 = (QV(1, 3) + QV(1, 4)) * 0.5;
+    const double w1 = (QV(2, 3) + QV(2, 4)) * 0.5, c1 = (QV(3, 3) + QV(3, 4)) * 0.5;
+    double a1, a2, a3;
+    if (unif) {
+      // average of two equal values, as the reference computes it
+      a1 = (B.M.u[q1] + B.M.u[q1]) * 0.5;
+      a2 = (B.M.u[q1 + 1] + B.M.u[q1 + 1]) * 0.5;
+      a3 = (B.M.u[q1 + 2] + B.M.u[q1 + 2]) * 0.5;
+    } else {
+      a1 = (QV(4, 3) + QV(4, 4)) * 0.5;
+      a2 = (QV(5, 3) + QV(5, 4)) * 0.5;
+      a3 = (QV(6, 3) + QV(6, 4)) * 0.5;
+    }
+#undef QV
+    char_basis(gamma, u1, v1, w1, c1, a1, a2, a3, Bs);
+  };
+  // characteristic flux h0(r) = h+(r) + h-(r) of face f: row r of S applied
+  // to the split fluxes on the scheme's stencil, then reconstructed
+  // (OCFD_inviscous_character.f90:250-282)
+  auto char_row = [&](const CharBasis &Bs, int f, int wl, int s, int scm, int r) {
     int a1h = S.ka1, b1h = S.kb1, a2h = S.ka2, b2h = S.kb2;
     if (HYB) {
       if (scm == 2) { a1h = -3; b1h = 3; a2h = -2; b2h = 4; }
       else if (scm == 3) { a1h = -2; b1h = 2; a2h = -1; b2h = 3; }
     }
-    CharBasis Bs;
-    if (scm != 1) {
-#define QV(q, k) sQ[((q) * NP + f + (k)) * TW + wl]
-      const double u1 = (QV(0, 3) + QV(0, 4)) * 0.5, v1 = (QV(1, 3) + QV(1, 4)) * 0.5;
-      const double w1 = (QV(2, 3) + QV(2, 4)) * 0.5, c1 = (QV(3, 3) + QV(3, 4)) * 0.5;
-      const double a1 = (QV(4, 3) + QV(4, 4)) * 0.5, a2 = (QV(5, 3) + QV(5, 4)) * 0.5;
-      const double a3 = (QV(6, 3) + QV(6, 4)) * 0.5;
-#undef QV
-      char_basis(gamma, u1, v1, w1, c1, a1, a2, a3, Bs);
+    double xp[8], xm[8], sr5[5];
+    char_S_row(Bs, r, sr5);
+    // xp(k) = sum over components c of S(r,c) * fp(c,k), summed in the
+    // reference's order c = 1..5; outside the scheme's stencil the value is 0
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      xp[k] = sr5[0] * FPV(0, k);
+      xm[k] = sr5[0] * FMV(0, k);
     }
-    double h0[5];
-    for (int r = 0; r < 5; r++) {
-      double xp[8], xm[8];
-      if (scm == 1) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) { xp[k] = FPV(r, k); xm[k] = FMV(r, k); }
-      } else {
-        double sr5[5];
-        char_S_row(Bs, r, sr5);
+    for (int c = 1; c < 5; c++) {
+      // keeps the scheduler from fetching all 80 stencil values at once
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-          xp[k] = 0.0;
-          xm[k] = 0.0;
-          if (k >= a1h + 3 && k <= b1h + 3)
-            xp[k] = sr5[0] * FPV(0, k) + sr5[1] * FPV(1, k) + sr5[2] * FPV(2, k) + sr5[3] * FPV(3, k) +
-                    sr5[4] * FPV(4, k);
-          if (k >= a2h + 3 && k <= b2h + 3)
-            xm[k] = sr5[0] * FMV(0, k) + sr5[1] * FMV(1, k) + sr5[2] * FMV(2, k) + sr5[3] * FMV(3, k) +
-                    sr5[4] * FMV(4, k);
-        }
+      for (int k = 0; k < 8; k++) {
+        xp[k] = xp[k] + sr5[c] * FPV(c, k);
+        xm[k] = xm[k] + sr5[c] * FMV(c, k);
       }
-      const double h1 = face_p_char<SCH>(S, xp, s, n, AXIS, scm);
-      const double h2 = face_m_char<SCH>(S, xm, s, n, AXIS, scm);
-      h0[r] = h1 + h2;
-    }
-    if (scm == 1) {
-      for (int r = 0; r < 5; r++) hh[r] = h0[r];
-    } else {
-      char_back(Bs, h0, hh);
     }
 #pragma unroll
-    for (int m = 0; m < 5; m++) sH[(m * NF + f) * TW + wl] = hh[m];
+    for (int k = 0; k < 8; k++) {
+      if (k < a1h + 3 || k > b1h + 3) xp[k] = 0.0;
+      if (k < a2h + 3 || k > b2h + 3) xm[k] = 0.0;
+    }
+    const double h1 = face_p_char<SCH>(S, xp, s, n, AXIS, scm);
+    const double h2 = face_m_char<SCH>(S, xm, s, n, AXIS, scm);
+    return h1 + h2;
+  };
+  // component m of a linear (hybrid UD7L) face: no projection
+  auto linear_comp = [&](int f, int wl, int s, int m) {
+    double xp[8], xm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { xp[k] = FPV(m, k); xm[k] = FMV(m, k); }
+    const double h1 = face_p_char<SCH>(S, xp, s, n, AXIS, 1);
+    const double h2 = face_m_char<SCH>(S, xm, s, n, AXIS, 1);
+    return h1 + h2;
+  };
+  // non-reflecting closure of component m (OCFD_inviscous_character.f90:295-309);
+  // point t is k = t - s + 3
+  auto closure_comp = [&](int f, int wl, int s, int m) {
+    if (s == 0)
+      return (2.0 * FPV(m, 4) + 5.0 * FPV(m, 5) - FPV(m, 6) + 2.0 * FMV(m, 6) - 7.0 * FMV(m, 5) +
+              11.0 * FMV(m, 4)) / 6.0;
+    return (2.0 * FPV(m, 1) - 7.0 * FPV(m, 2) + 11.0 * FPV(m, 3) + 2.0 * FMV(m, 3) + 5.0 * FMV(m, 2) -
+            FMV(m, 1)) / 6.0;
+  };
+  // all five components of face f, projected on the characteristic basis of
+  // the face (flux_charteric, OCFD_inviscous_character.f90:190-290)
+  auto face_ch = [&](int f, int wl, int s, int scm) {
+    CharBasis Bs;
+    basis_of(f, wl, Bs);
+    double h0[5], hh[5];
+    for (int r = 0; r < 5; r++) h0[r] = char_row(Bs, f, wl, s, scm, r);
+    char_back(Bs, h0, hh);
+#pragma unroll
+    for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = hh[m];
   };
   auto push = [&](int it, int scm) {
     if (scm == 2) sL[atomicAdd(&s_cnt[0], 1)] = it;
     else sL[NITEM - 1 - atomicAdd(&s_cnt[1], 1)] = it;
   };
+  auto scheme_of = [&](int f, int wl) {
+    return hybrid_index(dmax2(sR[lidx<AXIS, NP, TW>(0, f + 3, wl)], sR[lidx<AXIS, NP, TW>(0, f + 4, wl)]),
+                        S.hyb2, S.hyb3);
+  };
+  // face work item -> (component, face, line)
+  auto item_nc = [&](int it, int &m, int &f, int &wl) {
+    if (AXIS == 0) { f = it % NF; wl = (it / NF) % TW; } else { wl = it % TW; f = (it / TW) % NF; }
+    m = it / (TW * NF);
+  };
+  auto item_ch = [&](int it, int &f, int &wl) {
+    if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
+  };
+  const bool closure_lo = lo == 1 && sl == 0, closure_hi = hi == 1 && sr == 0;
   if (!CHAR) {
     for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
-      const int wl = it % TW;
-      const int f = (it / TW) % NF;
-      const int m = it / (TW * NF);
+      int m, f, wl;
+      item_nc(it, m, f, wl);
       const int s = s0 - 1 + f;
       if (s > n || w0 + wl > nw) continue;
       int scm = 0;
-      if (HYB) scm = hybrid_index(dmax2(sR[(f + 3) * TW + wl], sR[(f + 4) * TW + wl]), S.hyb2, S.hyb3);
+      if (HYB) scm = scheme_of(f, wl);
       if (!HYB || scm == 1) face_nc(m, f, wl, s, scm);
       else push(it, scm);
     }
-  } else {
+    if (HYB) {
+      // second pass: the listed WENO7 faces, then the WENO5 faces, spread
+      // over all threads
+      __syncthreads();
+      const int n7 = s_cnt[0], n5 = s_cnt[1];
+      for (int q = tid; q < n7 + n5; q += NTHREADS) {
+        const int it = q < n7 ? sL[q] : sL[NITEM - 1 - (q - n7)];
+        int m, f, wl;
+        item_nc(it, m, f, wl);
+        face_nc(m, f, wl, s0 - 1 + f, q < n7 ? 2 : 3);
+      }
+    }
+  } else if (!HYB) {
     for (int it = tid; it < NF * TW; it += NTHREADS) {
-      const int wl = it % TW;
-      const int f = it / TW;
+      int f, wl;
+      item_ch(it, f, wl);
       const int s = s0 - 1 + f;
       if (s > n || w0 + wl > nw) continue;
-      if ((lo == 1 && sl == 0 && s == 0) || (hi == 1 && sr == 0 && s == n)) {
-        // non-reflecting closure (OCFD_inviscous_character.f90:295-309); point t is k = t - s + 3
-        for (int m = 0; m < 5; m++) {
-          double hv;
-          if (s == 0)
-            hv = (2.0 * FPV(m, 4) + 5.0 * FPV(m, 5) - FPV(m, 6) + 2.0 * FMV(m, 6) - 7.0 * FMV(m, 5) +
-                  11.0 * FMV(m, 4)) / 6.0;
-          else
-            hv = (2.0 * FPV(m, 1) - 7.0 * FPV(m, 2) + 11.0 * FPV(m, 3) + 2.0 * FMV(m, 3) + 5.0 * FMV(m, 2) -
-                  FMV(m, 1)) / 6.0;
-          sH[(m * NF + f) * TW + wl] = hv;
-        }
+      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+        for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
         continue;
       }
-      int scm = 0;
-      if (HYB) scm = hybrid_index(dmax2(sR[(f + 3) * TW + wl], sR[(f + 4) * TW + wl]), S.hyb2, S.hyb3);
-      if (!HYB || scm == 1) face_ch(f, wl, s, scm);
-      else push(it, scm);
+      face_ch(f, wl, s, 0);
     }
-  }
-  if (HYB) {
-    // second pass: the listed WENO7 faces, then the WENO5 faces, spread over
-    // all threads
-    __syncthreads();
-    const int n7 = s_cnt[0], n5 = s_cnt[1];
-    for (int q = tid; q < n7 + n5; q += NTHREADS) {
-      const int it = q < n7 ? sL[q] : sL[NITEM - 1 - (q - n7)];
-      const int scm = q < n7 ? 2 : 3;
-      if (!CHAR) {
-        const int wl = it % TW, f = (it / TW) % NF, m = it / (TW * NF);
-        face_nc(m, f, wl, s0 - 1 + f, scm);
-      } else {
-        const int wl = it % TW, f = it / TW;
-        face_ch(f, wl, s0 - 1 + f, scm);
+  } else {
+    // hybrid, characteristic: linear and closure faces per component; the
+    // WENO faces are listed once and then computed one characteristic row
+    // per thread, followed by the back-projection one face per thread
+    for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
+      int m, f, wl;
+      item_nc(it, m, f, wl);
+      const int s = s0 - 1 + f;
+      if (s > n || w0 + wl > nw) continue;
+      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+        sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
+        continue;
       }
+      const int scm = scheme_of(f, wl);
+      if (scm == 1) sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+      else if (m == 0) push(it, scm);
+    }
+    __syncthreads();
+    const int n7 = s_cnt[0], n5 = s_cnt[1], nl = n7 + n5;
+    for (int q = tid; q < 5 * nl; q += NTHREADS) {
+      const int lf = q / 5, r = q - 5 * lf;
+      const int it = lf < n7 ? sL[lf] : sL[NITEM - 1 - (lf - n7)];
+      int f, wl;
+      item_ch(it, f, wl);
+      CharBasis Bs;
+      basis_of(f, wl, Bs);
+      sH[lidx<AXIS, NF, TW>(r, f, wl)] = char_row(Bs, f, wl, s0 - 1 + f, lf < n7 ? 2 : 3, r);
+    }
+    __syncthreads();
+    for (int lf = tid; lf < nl; lf += NTHREADS) {
+      const int it = lf < n7 ? sL[lf] : sL[NITEM - 1 - (lf - n7)];
+      int f, wl;
+      item_ch(it, f, wl);
+      CharBasis Bs;
+      basis_of(f, wl, Bs);
+      double h0[5], hh[5];
+#pragma unroll
+      for (int r = 0; r < 5; r++) h0[r] = sH[lidx<AXIS, NF, TW>(r, f, wl)];
+      char_back(Bs, h0, hh);
+#pragma unroll
+      for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = hh[m];
     }
   }
 #undef FPV
@@ -235,47 +371,50 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
 
   // ---- phase 3: flux difference into du ----
   const double hinv = B.P.hinv[AXIS];
-  for (int it = tid; it < 5 * TL * TW; it += NTHREADS) {
+#pragma unroll
+  for (int r = 0; r < NI3; r++) {
     int sl_, wl, m;
-    if (AXIS == 0) { sl_ = it % TL; wl = (it / TL) % TW; m = it / (TL * TW); }
-    else { wl = it % TW; sl_ = (it / TW) % TL; m = it / (TL * TW); }
+    if (!p3(r, sl_, wl, m)) continue;
     const int s = s0 + sl_, w = w0 + wl;
-    if (s > n || w > nw) continue;
     const int f = sl_ + 1;
     double val;
     if (!CHAR) {
-      val = (sH[(m * NF + f) * TW + wl] - sH[(m * NF + f - 1) * TW + wl] + sH[((5 + m) * NF + f) * TW + wl] -
-             sH[((5 + m) * NF + f - 1) * TW + wl]) * hinv;
+      val = (sH[lidx<AXIS, NF, TW>(m, f, wl)] - sH[lidx<AXIS, NF, TW>(m, f - 1, wl)] +
+             sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] - sH[lidx<AXIS, NF, TW>(5 + m, f - 1, wl)]) * hinv;
     } else {
-      val = (sH[(m * NF + f) * TW + wl] - sH[(m * NF + f - 1) * TW + wl]) * hinv;
+      val = (sH[lidx<AXIS, NF, TW>(m, f, wl)] - sH[lidx<AXIS, NF, TW>(m, f - 1, wl)]) * hinv;
     }
     const int64_t q = TI::iidx(g, s, w, c) + (int64_t)m * g.ni;
     if (mode == INV_ASSIGN) {
       B.du[q] = val;
     } else if (mode == INV_ACCUM) {
-      B.du[q] = B.du[q] + val;
+      B.du[q] = pdu[r] + val;
     } else {
       const int64_t h = TI::hidx(g, s, w, c);
-      double r = -(B.du[q] + val) * B.M.get(M_AJAC, h);
+      double rr = -(pdu[r] + val) * B.M.get(M_AJAC, h);
       if (B.P.if_mass_force == 1 && m > 0) {
         // OCFD_inviscous.f90:189-201
         const double *mf = B.P.mass_force;
-        if (m < 4) r = r + B.d[h] * mf[m - 1];
-        else r = r + B.d[h] * (B.u[h] * mf[0] + B.v[h] * mf[1] + B.w[h] * mf[2]);
+        if (m < 4) rr = rr + B.d[h] * mf[m - 1];
+        else rr = rr + B.d[h] * (B.u[h] * mf[0] + B.v[h] * mf[1] + B.w[h] * mf[2]);
       }
-      B.du[q] = r;
+      B.du[q] = rr;
     }
   }
 }
 
+template <int TL, int TW, bool CHAR, bool HYB>
+constexpr size_t sweep_lds(int nq) {
+  return sizeof(double) * (10 * (TL + 8) * TW + (CHAR ? nq * (TL + 8) * TW : 0) + (HYB ? (TL + 8) * TW : 0) +
+                           (CHAR ? 5 : 10) * (TL + 1) * TW) +
+         (HYB ? sizeof(int) * (CHAR ? (TL + 1) * TW : 5 * (TL + 1) * TW) : 0);
+}
+
 template <int AXIS, int SCH, bool CHAR, int TL, int TW>
 hipError_t launch_one(const Block &B, int mode, hipStream_t st) {
-  constexpr int NP = TL + 8, NF = TL + 1;
   constexpr bool HYB = (SCH == SCH_HYBRID);
-  constexpr size_t lds = sizeof(double) * (10 * NP * TW + (CHAR ? 7 * NP * TW : 0) + (HYB ? NP * TW : 0) +
-                                           (CHAR ? 5 : 10) * NF * TW) +
-                         (HYB ? sizeof(int) * (CHAR ? NF * TW : 5 * NF * TW) : 0);
-  static_assert(lds <= 64 * 1024, "tile too large for LDS");
+  static_assert(sweep_lds<TL, TW, CHAR, HYB>(7) <= 64 * 1024, "tile too large for LDS");
+  const size_t lds = sweep_lds<TL, TW, CHAR, HYB>(B.M.uniform ? 4 : 7);
   const Geom &g = B.g;
   const int n = g.n(AXIS);
   const int nw = AXIS == 0 ? g.ny : g.nx;
@@ -303,8 +442,8 @@ hipError_t dispatch_scheme(const Block &B, int mode, hipStream_t st) {
 }  // namespace
 
 hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, hipStream_t st) {
-  (void)tile;
   const bool ch = B.S.is_char != 0;
+  (void)tile;
   if (axis == 0) return ch ? dispatch_scheme<0, true>(B, mode, st) : dispatch_scheme<0, false>(B, mode, st);
   if (axis == 1) return ch ? dispatch_scheme<1, true>(B, mode, st) : dispatch_scheme<1, false>(B, mode, st);
   return ch ? dispatch_scheme<2, true>(B, mode, st) : dispatch_scheme<2, false>(B, mode, st);

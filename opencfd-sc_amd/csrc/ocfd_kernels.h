@@ -19,9 +19,9 @@ struct Block {
 
 enum InvMode : int { INV_ASSIGN = 0, INV_ACCUM = 1, INV_FINAL = 2 };
 
-// Tile shape of one sweep launch: TL cells along the sweep, TW lines across.
+// Launch shape of one sweep (0: the built-in tile of each axis).
 struct SweepTile {
-  int tl, tw;
+  int shape;
 };
 
 // ---- launchers (return hipError_t of the launch) ----
