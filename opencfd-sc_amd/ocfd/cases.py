@@ -245,7 +245,7 @@ def random_case(shape=(10, 12, 16), seed=1, scheme="WENO5", split=1, char=0, vis
                 iperiodic=(1, 1, 1), ma=0.8, re=300.0, noise=2e-3, jump=0.0,
                 grid="box", scheme_boundary=(0, 0, 0, 0, 0, 0), ghost_cell=(0,) * 6,
                 bc="BC_NONE", bc_para=None, filters=(), hybrid=(0.02, 0.1),
-                mass_force=None, dt=None) -> Case:
+                mass_force=None, dt=None, sensor=1) -> Case:
     """Seeded, non-cubic test case exercising nonlinear weights (smooth field
     plus white noise), optionally a density/temperature jump (shock sensor),
     a curvilinear grid and non-periodic boundaries."""
@@ -294,7 +294,7 @@ def random_case(shape=(10, 12, 16), seed=1, scheme="WENO5", split=1, char=0, vis
                      ma=ma, re=re, grid_type=gtype, scheme_boundary=tuple(scheme_boundary),
                      ghost_cell=tuple(ghost_cell), boundary_condition=bc,
                      bc_para=dict(bc_para or {}), filters=[list(f) for f in filters],
-                     hybrid_para={1: 1.0, 2: hybrid[0], 3: hybrid[1], 4: 0.0},
+                     hybrid_para={1: float(sensor), 2: hybrid[0], 3: hybrid[1], 4: 0.0},
                      periodic_span=tuple(_periodic_spans(iperiodic, *lens)))
     if mass_force is not None:
         cfg.if_mass_force = 1

@@ -15,7 +15,7 @@ reference's outputs:
   hist      per step: Istep, tt, E, K, S (show_flow_msg), KE, enstrophy (ana_box)
 
 Usage (needs the reference built by `make -C oracle ref probe`):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -63,6 +63,8 @@ def golden_cases():
                                 filters=F2),
         "blunt_massforce": C(seed=15, scheme="OMP6", split=1, char=1, grid="curv", iperiodic=(1, 0, 1),
                              bc="BC_BLUNT2D", mass_force=(0.1, -0.2, 0.05)),
+        "hyb_sensor2_curv": C(seed=16, scheme="HYBRID", split=2, char=1, jump=0.6, grid="curv",
+                              iperiodic=(1, 1, 0), hybrid=(1.0, 3.0), sensor=2),
     }
 
 
@@ -78,7 +80,10 @@ def main():
         raise SystemExit("build the reference first: make -C oracle ref probe")
     tmp = tempfile.mkdtemp(prefix="ocfd_golden_")
     try:
+        only = set(sys.argv[1:])
         for name, case in golden_cases().items():
+            if only and name not in only:
+                continue
             ref = pyoracle.run_probe(case, os.path.join(tmp, name), nextra=2)
             L = case.cfg.lap
             out = {
