@@ -1,0 +1,282 @@
+! ocfd_dropin.f90 -- the reference solver's hot-path routines served by
+! libocfd_hip.so, for the reference driver NS_solver (OCFD_NS_Solver.f90:5-90)
+! left unchanged.
+!
+! The executable is linked from the reference's own objects with
+! `ld --wrap=<symbol>` for every routine below: a call from NS_solver to
+! du_inviscous_ lands in __wrap_du_inviscous_ here, and __real_<symbol> is the
+! reference's original.  The state lives on the GPU from the first OCFD_bc
+! (end of initialisation, OCFD_NS_Solver.f90:15) on; host arrays are refreshed
+! from the device only where host code reads them (show_flow_msg,
+! OCFD_analysis, save_flow_data).
+!
+!   exchange_boundary_xyz(d..T)  -> ocfd_exchange_primitives (OCFD_mpi.f90:183-196)
+!   comput_Rhybrid               -> ocfd_comput_rhybrid      (OCFD_SetHybrid.f90:2-152)
+!   du_inviscous                 -> ocfd_du_inviscous        (OCFD_inviscous.f90:53-203)
+!   du_inviscous_Character       -> ocfd_du_inviscous_character (OCFD_inviscous_character.f90:3-159)
+!   du_viscous                   -> ocfd_du_viscous          (OCFD_viscous.f90:29-131)
+!   OCFD_time_adv_RK3(KRK)       -> ocfd_save_fn (KRK=1: the fn=f loop of
+!                                   OCFD_NS_Solver.f90:23-31) + ocfd_time_adv_rk3
+!   comput_duvwT                 -> ocfd_comput_duvwt        (OCFD_Time_Adv.f90:40-63)
+!   OCFD_bc                      -> ocfd_bc                  (OCFD_BC.f90:2-99)
+!   filtering(f)                 -> ocfd_advance_clock + ocfd_filtering (OCFD_filtering.f90:4-70)
+module ocfd_dropin_state
+   use iso_c_binding
+   implicit none
+   type(c_ptr), save :: hdev = c_null_ptr
+   ! .true. while host code that reads the state runs (its own halo
+   ! exchanges must then act on the host arrays)
+   logical, save :: host_section = .false.
+end module ocfd_dropin_state
+
+! Creates the device block from the reference's parameters and uploads the
+! metrics and the initial state (after init3d and the first OCFD_bc).
+subroutine ocfd_dropin_attach()
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   type(ocfd_config) :: cfg
+   type(c_ptr) :: mets(19)
+   integer :: dev
+   character(len=16) :: env
+   integer :: st
+
+   if (np_size /= 1) then
+      print *, 'libocfd_hip drop-in: one MPI rank per block only (np_size =', np_size, ')'
+      stop
+   end if
+   cfg%nx = nx_global; cfg%ny = ny_global; cfg%nz = nz_global; cfg%lap = LAP
+   cfg%iperiodic = [Para%Iperiodic_X, Para%Iperiodic_Y, Para%Iperiodic_Z]
+   cfg%scheme_invis = Scheme%Scheme_Invis
+   cfg%scheme_vis = Scheme%Scheme_Vis
+   cfg%flux_splitting = Para%Flux_Splitting
+   cfg%if_scheme_character = Para%IF_Scheme_Character
+   cfg%if_viscous = Para%IF_Viscous
+   cfg%ibc = Para%IBC
+   cfg%if_mass_force = Para%IF_Mass_Force
+   cfg%scheme_boundary = Scheme%Scheme_boundary
+   cfg%ghost_cell = Para%Ghost_cell
+   cfg%nfiltering = Para%Nfiltering
+   cfg%istep_show = Para%Istep_Show
+   cfg%precision = OCFD_PREC_FP64
+   cfg%metric_mode = OCFD_METRIC_FIELD
+   cfg%reserved_i = 0
+   cfg%ma = Para%Ma; cfg%re = Para%Re; cfg%gamma = Para%gamma; cfg%pr = Para%Pr
+   cfg%ref_amu_t0 = Para%Ref_Amu_T0; cfg%dt = Para%dt
+   cfg%ud7l_diss = Scheme%UD7L_Diss
+   cfg%mass_force = Para%Mass_Force
+   cfg%bc_para = Para%BC_para
+   cfg%hybrid_para = Scheme%Hybrid_para
+   cfg%filter_para = Para%Filter
+   cfg%reserved_d = 0.d0
+
+   dev = 0
+   call get_environment_variable('OCFD_HIP_DEVICE', env, status=st)
+   if (st == 0) read (env, *) dev
+   call ocfd_check(ocfd_create(cfg, int(dev, c_int), hdev), 'ocfd_create')
+
+   mets = [ocfd_addr(Akx), ocfd_addr(Aky), ocfd_addr(Akz), ocfd_addr(Aix), ocfd_addr(Aiy), &
+           ocfd_addr(Aiz), ocfd_addr(Asx), ocfd_addr(Asy), ocfd_addr(Asz), ocfd_addr(Ajac), &
+           ocfd_addr(Akx1), ocfd_addr(Aky1), ocfd_addr(Akz1), ocfd_addr(Aix1), ocfd_addr(Aiy1), &
+           ocfd_addr(Aiz1), ocfd_addr(Asx1), ocfd_addr(Asy1), ocfd_addr(Asz1)]
+   call ocfd_check(ocfd_upload_metrics(hdev, mets), 'ocfd_upload_metrics')
+   call ocfd_check(ocfd_upload_primitives(hdev, d, u, v, w, T, int(Istep, c_int32_t), tt), &
+                   'ocfd_upload_primitives')
+   call ocfd_check(ocfd_upload_f(hdev, f), 'ocfd_upload_f')
+   if (my_id .eq. 0) print *, 'libocfd_hip: block on GPU', dev
+end subroutine ocfd_dropin_attach
+
+! Device state -> host arrays d,u,v,w,T (halo'd) and f.
+subroutine ocfd_dropin_download()
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   call ocfd_check(ocfd_download_primitives(hdev, d, u, v, w, T), 'ocfd_download_primitives')
+   call ocfd_check(ocfd_download_f(hdev, f), 'ocfd_download_f')
+end subroutine ocfd_dropin_download
+
+subroutine wrap_ocfd_bc() bind(C, name="__wrap_ocfd_bc_")
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   interface
+      subroutine real_ocfd_bc() bind(C, name="__real_ocfd_bc_")
+      end subroutine
+   end interface
+   if (.not. c_associated(hdev)) then
+      ! initial boundary condition on the host, then the state moves to the GPU
+      call real_ocfd_bc()
+      call ocfd_dropin_attach()
+   else if (host_section) then
+      call real_ocfd_bc()
+   else
+      call ocfd_check(ocfd_bc(hdev), 'ocfd_bc')
+   end if
+end subroutine wrap_ocfd_bc
+
+subroutine wrap_exchange_boundary_xyz(a) bind(C, name="__wrap_exchange_boundary_xyz_")
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   real(c_double), target :: a(*)
+   type(c_ptr) :: pa
+   interface
+      subroutine real_exchange_boundary_xyz(a) bind(C, name="__real_exchange_boundary_xyz_")
+         import :: c_double
+         real(c_double) :: a(*)
+      end subroutine
+   end interface
+   if (.not. c_associated(hdev) .or. host_section) then
+      call real_exchange_boundary_xyz(a)
+      return
+   end if
+   pa = c_loc(a(1))
+   if (c_associated(pa, ocfd_addr(d))) then
+      ! NS_solver exchanges d, u, v, w, T in this order: all five on the first
+      call ocfd_check(ocfd_exchange_primitives(hdev), 'ocfd_exchange_primitives')
+   else if (c_associated(pa, ocfd_addr(u)) .or. c_associated(pa, ocfd_addr(v)) .or. &
+            c_associated(pa, ocfd_addr(w)) .or. c_associated(pa, ocfd_addr(T))) then
+      continue
+   else
+      call real_exchange_boundary_xyz(a)
+   end if
+end subroutine wrap_exchange_boundary_xyz
+
+subroutine wrap_comput_rhybrid() bind(C, name="__wrap_comput_rhybrid_")
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   integer(c_int64_t), target :: counts(4)
+   if (mod(Istep, Para%Istep_show) .eq. 0) then
+      call ocfd_check(ocfd_comput_rhybrid(hdev, c_loc(counts)), 'ocfd_comput_rhybrid')
+      ! the message of OCFD_SetHybrid.f90:134
+      if (my_id .eq. 0) print *, "---the percent of 3 schemes (linear ,WENO7,WENO5 are: ", &
+         counts(1:3)/(1.d0*counts(4))
+   else
+      call ocfd_check(ocfd_comput_rhybrid(hdev, c_null_ptr), 'ocfd_comput_rhybrid')
+   end if
+end subroutine wrap_comput_rhybrid
+
+subroutine wrap_du_inviscous() bind(C, name="__wrap_du_inviscous_")
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   call ocfd_check(ocfd_du_inviscous(hdev), 'ocfd_du_inviscous')
+end subroutine wrap_du_inviscous
+
+subroutine wrap_du_inviscous_character() bind(C, name="__wrap_du_inviscous_character_")
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   call ocfd_check(ocfd_du_inviscous_character(hdev), 'ocfd_du_inviscous_character')
+end subroutine wrap_du_inviscous_character
+
+subroutine wrap_du_viscous() bind(C, name="__wrap_du_viscous_")
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   call ocfd_check(ocfd_du_viscous(hdev), 'ocfd_du_viscous')
+end subroutine wrap_du_viscous
+
+subroutine wrap_ocfd_time_adv_rk3(krk) bind(C, name="__wrap_ocfd_time_adv_rk3_")
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   integer(c_int32_t) :: krk
+   ! fn = f of the step start (OCFD_NS_Solver.f90:23-31) happens on the device
+   if (krk == 1) call ocfd_check(ocfd_save_fn(hdev), 'ocfd_save_fn')
+   call ocfd_check(ocfd_time_adv_rk3(hdev, krk), 'ocfd_time_adv_rk3')
+end subroutine wrap_ocfd_time_adv_rk3
+
+subroutine wrap_comput_duvwt() bind(C, name="__wrap_comput_duvwt_")
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   integer(c_int) :: nneg
+   nneg = ocfd_comput_duvwt(hdev)
+   call ocfd_check(nneg, 'ocfd_comput_duvwt')
+   if (nneg > 0) then
+      ! handle_NegativeT (OCFD_handle_NegativeT.f90:1-17): cells repaired on
+      ! the device; the run stops beyond 10 of them
+      print *, "Negative T found !", nneg, " cells (repaired)"
+      if (nneg > 10) then
+         print *, "Number of Negative Temperature Points >  Limit, STOP !"
+         stop
+      end if
+   end if
+end subroutine wrap_comput_duvwt
+
+subroutine wrap_filtering(fa) bind(C, name="__wrap_filtering_")
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   real(c_double) :: fa(*)
+   integer(c_int32_t) :: istep_dev
+   real(c_double) :: tt_dev
+   ! NS_solver advances Istep and tt right before filtering(f)
+   ! (OCFD_NS_Solver.f90:64-67); the device clock follows here
+   call ocfd_check(ocfd_advance_clock(hdev), 'ocfd_advance_clock')
+   call ocfd_check(ocfd_get_time(hdev, istep_dev, tt_dev), 'ocfd_get_time')
+   if (istep_dev /= Istep) then
+      print *, 'libocfd_hip drop-in: device step', istep_dev, ' differs from Istep', Istep
+      stop
+   end if
+   call ocfd_check(ocfd_filtering(hdev), 'ocfd_filtering')
+end subroutine wrap_filtering
+
+subroutine wrap_show_flow_msg(wall_time) bind(C, name="__wrap_show_flow_msg_")
+   use ocfd_dropin_state
+   implicit none
+   real(c_double) :: wall_time
+   interface
+      subroutine real_show_flow_msg(wall_time) bind(C, name="__real_show_flow_msg_")
+         import :: c_double
+         real(c_double) :: wall_time
+      end subroutine
+   end interface
+   call ocfd_dropin_download()
+   host_section = .true.
+   call real_show_flow_msg(wall_time)
+   host_section = .false.
+end subroutine wrap_show_flow_msg
+
+subroutine wrap_ocfd_analysis() bind(C, name="__wrap_ocfd_analysis_")
+   use flow_data
+   use ocfd_dropin_state
+   implicit none
+   integer :: m, kstep
+   logical :: due
+   interface
+      subroutine real_ocfd_analysis() bind(C, name="__real_ocfd_analysis_")
+      end subroutine
+   end interface
+   ! the schedule test of OCFD_analysis (OCFD_ANA.f90:9-14)
+   due = .false.
+   do m = 1, Para%ANA_Number
+      kstep = nint(Para%ANA_Para(2, m))
+      if (mod(Istep, kstep) .eq. 0) due = .true.
+   end do
+   if (due) call ocfd_dropin_download()
+   host_section = .true.
+   call real_ocfd_analysis()
+   host_section = .false.
+end subroutine wrap_ocfd_analysis
+
+subroutine wrap_save_flow_data() bind(C, name="__wrap_save_flow_data_")
+   use ocfd_dropin_state
+   implicit none
+   interface
+      subroutine real_save_flow_data() bind(C, name="__real_save_flow_data_")
+      end subroutine
+   end interface
+   if (c_associated(hdev)) call ocfd_dropin_download()
+   host_section = .true.
+   call real_save_flow_data()
+   host_section = .false.
+end subroutine wrap_save_flow_data

@@ -4,7 +4,13 @@ reference and against the C oracle.
 Tolerances (FP64 path, per field):
   per-routine outputs and whole steps on the golden cases: bitwise, i.e.
     relative L2 == 0 (the kernels keep the reference's operation order and
-    are built without FMA contraction);
+    are built without FMA contraction); the one exception is Rhybrid of the
+    pressure-gradient sensor (Hybrid_para(1) = 2), which divides by the mean
+    of |grad p| over the block: the device sums in parallel, not in the
+    reference's i-j-k loop order, so Rhybrid may differ in the last bit
+    (relative L2 <= 1e-14); Rhybrid is only compared with the thresholds, so
+    everything downstream stays bitwise unless a value sits within an ulp of
+    a threshold;
   100-step TGV and isotropic runs against the oracle: relative L2 <= 1e-12
     per field, kinetic-energy / enstrophy histories <= 1e-10 relative.
 """
@@ -43,7 +49,10 @@ def test_golden_per_routine_bitwise(name):
         b.exchange_primitives()
         if cfg.scheme_invis.upper() == "HYBRID":
             b.comput_rhybrid()
-            assert_same(interior(b.download_rhybrid()), ref["rhyb"], "Rhybrid")
+            if cfg.hybrid_para.get(1, 1.0) == 2.0:
+                assert rel_l2(interior(b.download_rhybrid()), ref["rhyb"]) <= 1e-14
+            else:
+                assert_same(interior(b.download_rhybrid()), ref["rhyb"], "Rhybrid")
         if cfg.if_scheme_character:
             b.du_inviscous_character()
         else:
