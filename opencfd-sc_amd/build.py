@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libocfd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ocfd_inviscid.hip", "ocfd_viscous.hip", "ocfd_misc.hip", "ocfd_api.hip"]
+SOURCES = ["ocfd_inviscid.hip", "ocfd_inviscid_mixed.hip", "ocfd_hybrid.hip", "ocfd_viscous.hip", "ocfd_misc.hip", "ocfd_api.hip"]
 FLAGS = [
     "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -49,6 +50,8 @@ struct ocfd_handle {
   int istep = 0;
   double tt = 0.0;
   bool prof = false;
+  bool use_graph = true;           // replay whole steps from a captured hipGraph
+  hipGraphExec_t gexec = nullptr;  // one time step (three stages), captured on first use
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
@@ -124,6 +127,7 @@ int set_params(ocfd_handle *h) {
   for (int q = 0; q < 6; q++)
     if (c.scheme_boundary[q] < -1 || c.scheme_boundary[q] > 1) return fail("Scheme_boundary must be -1, 0 or 1");
   Block &B = h->B;
+  B.prec = c.precision == OCFD_PREC_MIXED ? OCFD_PREC_MIXED_CODE : OCFD_PREC_FP64_CODE;
   Geom &g = B.g;
   g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
   g.sx = c.nx + 2 * LAP; g.sy = c.ny + 2 * LAP; g.sz = c.nz + 2 * LAP;
@@ -156,6 +160,7 @@ int set_params(ocfd_handle *h) {
   P.h[1] = 1.0 / ((double)c.ny - 1.0);
   P.h[2] = 1.0 / ((double)c.nz - 1.0);
   for (int a = 0; a < 3; a++) P.hinv[a] = 1.0 / P.h[a];
+  for (int a = 0; a < 3; a++) P.dc[a] = make_dcoef(P.h[a]);
   P.cv = 1.0 / (c.gamma * (c.gamma - 1.0) * c.ma * c.ma);
   P.cp = P.cv * c.gamma;
   P.if_mass_force = c.if_mass_force;
@@ -173,6 +178,11 @@ __global__ void k_fill(double *a, int64_t n, double v) {
 hipError_t fill(double *a, int64_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, v);
   return hipGetLastError();
+}
+
+__global__ void k_fill_int(int *a, int64_t n, int v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
 }
 
 __global__ void k_zero_int(int *a, int n) {
@@ -208,6 +218,13 @@ double *prims_of(ocfd_handle *h, int q) {
 int inviscid(ocfd_handle *h, bool ch) {
   Block B = h->B;
   B.S.is_char = ch ? 1 : 0;
+  if (B.hcount) {
+    // hybrid: the WENO faces of the three sweeps first
+    if ((h->cfg.if_scheme_character != 0) != ch)
+      return fail("hybrid scheme: the inviscid routine must match IF_Scheme_Character");
+    for (int a = 0; a < 3; a++)
+      if (launch(h, "weno_faces", [&] { return launch_weno_faces(B, a, h->st); })) return -1;
+  }
   static const char *nm[2][3] = {{"inviscid_x", "inviscid_y", "inviscid_z"},
                                  {"inviscid_char_x", "inviscid_char_y", "inviscid_char_z"}};
   const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
@@ -243,7 +260,10 @@ int sensor(ocfd_handle *h) {
                        jb, je, kb, ke);
   }
   double *r[1] = {h->B.rhyb};
-  return launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, r, 1, h->st); });
+  if (launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, r, 1, h->st); })) return -1;
+  // the faces that use WENO7 / WENO5 until the next sensor evaluation
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->B.hcount, 4);
+  return launch(h, "hyb_list", [&] { return launch_hybrid_lists(h->B, h->st); });
 }
 
 int filtering(ocfd_handle *h) {
@@ -301,6 +321,7 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { delete h; return fail("no HIP device"); }
   if (device < 0 || device >= ndev) { delete h; return fail("device index out of range"); }
   h->device = device;
+  h->use_graph = std::getenv("OCFD_NO_GRAPH") == nullptr;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&h->st) != hipSuccess) {
     delete h;
     return fail("cannot create HIP stream");
@@ -313,6 +334,23 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   if (cfg->scheme_invis == SCH_HYBRID) rc |= alloc(&B.rhyb, hb);
   if (cfg->if_viscous == 1) rc |= alloc(&B.ev, 12 * (size_t)B.g.ni * sizeof(double));
   if (cfg->nfiltering > 0) rc |= alloc(&h->fbuf, fb);
+  rc |= alloc(&B.trash, 16 * 256 * sizeof(double));
+  for (int a = 0; a < 3; a++) {
+    B.hlist[a] = B.hmap[a] = nullptr;
+    B.wflux[a] = nullptr;
+  }
+  B.hcount = nullptr;
+  if (cfg->scheme_invis == SCH_HYBRID) {
+    // worst case: every face of every axis uses WENO
+    const int nh = cfg->if_scheme_character ? 5 : 10;
+    const int64_t nfa[3] = {TileIndex<0>::nfaces(B.g), TileIndex<1>::nfaces(B.g), TileIndex<2>::nfaces(B.g)};
+    for (int a = 0; a < 3; a++) {
+      rc |= alloc((double **)&B.hlist[a], (size_t)nfa[a] * sizeof(int));
+      rc |= alloc((double **)&B.hmap[a], (size_t)nfa[a] * sizeof(int));
+      rc |= alloc(&B.wflux[a], (size_t)nfa[a] * nh * sizeof(double));
+    }
+    rc |= alloc((double **)&B.hcount, 4 * sizeof(int));
+  }
   B.M.uniform = cfg->metric_mode == OCFD_METRIC_UNIFORM;
   if (!B.M.uniform)
     for (int m = 0; m < M_N; m++) rc |= alloc(&h->met[m], hb);
@@ -340,6 +378,14 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   (void)fill(B.f, 5 * B.g.ni, 0.0, h->st);
   (void)fill(B.fn, 5 * B.g.ni, 0.0, h->st);
   if (B.ev) (void)fill(B.ev, 12 * B.g.ni, 0.0, h->st);
+  if (B.hcount) {
+    // no face listed until the first comput_Rhybrid
+    for (int a = 0; a < 3; a++) {
+      const int64_t nfa = a == 0 ? TileIndex<0>::nfaces(B.g) : (a == 1 ? TileIndex<1>::nfaces(B.g) : TileIndex<2>::nfaces(B.g));
+      hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nfa + 255) / 256)), dim3(256), 0, h->st, B.hmap[a], nfa, -1);
+    }
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, B.hcount, 4);
+  }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
   if (hipStreamSynchronize(h->st) != hipSuccess) {
     ocfd_destroy(h);
@@ -349,14 +395,23 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   return 0;
 }
 
+void invalidate_graph(ocfd_handle *h);
+
 int ocfd_destroy(ocfd_handle *h) {
   if (!h) return 0;
   (void)hipSetDevice(h->device);
   if (h->st) (void)hipStreamSynchronize(h->st);
+  invalidate_graph(h);
   Block &B = h->B;
-  double *ps[] = {B.d, B.u, B.v, B.w, B.T, B.f, B.fn, B.du, B.rhyb, B.ev, h->fbuf, h->work, h->wout};
+  double *ps[] = {B.d, B.u, B.v, B.w, B.T, B.f, B.fn, B.du, B.rhyb, B.ev, B.trash, h->fbuf, h->work, h->wout};
   for (double *p : ps)
     if (p) (void)hipFree(p);
+  for (int a = 0; a < 3; a++) {
+    if (B.hlist[a]) (void)hipFree(B.hlist[a]);
+    if (B.hmap[a]) (void)hipFree(B.hmap[a]);
+    if (B.wflux[a]) (void)hipFree(B.wflux[a]);
+  }
+  if (B.hcount) (void)hipFree(B.hcount);
   for (int m = 0; m < M_N; m++)
     if (h->met[m]) (void)hipFree(h->met[m]);
   if (h->neg) (void)hipFree(h->neg);
@@ -410,6 +465,7 @@ int ocfd_set_uniform_metrics(ocfd_handle *h, const double m[10]) {
   if (!h->B.M.uniform) return fail("ocfd_set_uniform_metrics needs OCFD_METRIC_UNIFORM");
   for (int q = 0; q < 10; q++) h->B.M.u[q] = m[q];
   for (int q = 0; q < 9; q++) h->B.M.u[M_AKX1 + q] = m[q] / m[M_AJAC];
+  invalidate_graph(h);  // the constants travel in the kernel arguments
   return 0;
 }
 
@@ -536,30 +592,71 @@ int ocfd_advance_clock(ocfd_handle *h) {
 }
 
 // ---- whole steps -----------------------------------------------------------
-int ocfd_step(ocfd_handle *h, int32_t nsteps) {
+namespace {
+// the three stages of one time step (OCFD_NS_Solver.f90:33-60)
+int step_body(ocfd_handle *h) {
   const ocfd_config &c = h->cfg;
   const bool hyb = c.scheme_invis == SCH_HYBRID;
   const bool ch = c.if_scheme_character != 0;
+  for (int krk = 1; krk <= 3; krk++) {
+    if (exchange_prims(h)) return -1;
+    if (krk == 1 && hyb && sensor(h)) return -1;
+    if (inviscid(h, ch)) return -1;
+    if (c.if_viscous == 1 && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
+    Block B = h->B;
+    if (c.if_viscous != 1) B.ev = nullptr;
+    const int refresh = h->ibc == 0 ? 1 : 0;
+    if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
+      return -1;
+    if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+    if (h->ibc != 0) {
+      if (launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); })) return -1;
+    } else {
+      bool any = false;
+      for (int q = 0; q < 6; q++) any |= (h->ghost[q] == 1 || h->ghost[q] == 2);
+      if (any && launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, h->ghost, 0, h->st); })) return -1;
+    }
+  }
+  return 0;
+}
+
+// Records step_body into a graph (kernel launches only: no memset, copy or
+// host synchronisation happens inside it).
+int capture_step(ocfd_handle *h) {
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  const int rc = step_body(h);
+  const hipError_t e = hipStreamEndCapture(h->st, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    h->gexec = nullptr;
+    return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  return 0;
+}
+}  // namespace
+
+void invalidate_graph(ocfd_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
+}
+
+int ocfd_step(ocfd_handle *h, int32_t nsteps) {
+  const ocfd_config &c = h->cfg;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  // filtering decides per step on the host; per-kernel timing needs eager launches
+  const bool graph = h->use_graph && !h->prof && c.nfiltering == 0;
   for (int n = 0; n < nsteps; n++) {
-    for (int krk = 1; krk <= 3; krk++) {
-      if (exchange_prims(h)) return -1;
-      if (krk == 1 && hyb && sensor(h)) return -1;
-      if (inviscid(h, ch)) return -1;
-      if (c.if_viscous == 1 && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
-      Block B = h->B;
-      if (c.if_viscous != 1) B.ev = nullptr;
-      const int refresh = h->ibc == 0 ? 1 : 0;
-      if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
-        return -1;
-      if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
-      if (h->ibc != 0) {
-        if (launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); })) return -1;
-      } else {
-        bool any = false;
-        for (int q = 0; q < 6; q++) any |= (h->ghost[q] == 1 || h->ghost[q] == 2);
-        if (any && launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, h->ghost, 0, h->st); })) return -1;
-      }
+    if (graph) {
+      if (!h->gexec && capture_step(h)) return -1;
+      HIPCHK(hipGraphLaunch(h->gexec, h->st));
+    } else if (step_body(h)) {
+      return -1;
     }
     h->istep += 1;
     h->tt = h->tt + c.dt;

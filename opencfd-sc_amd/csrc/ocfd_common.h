@@ -58,6 +58,9 @@ struct Metrics {
 };
 
 // Scheme parameters derived by set_parameters (OCFD_readpara.f90:261-341).
+// Arithmetic of the right-hand side (values of OCFD_PREC_* in the C ABI).
+enum : int { OCFD_PREC_FP64_CODE = 0, OCFD_PREC_MIXED_CODE = 1 };
+
 struct SchemeParams {
   int scheme_invis, scheme_vis, flux_split, is_char;
   int ka1, kb1, ka2, kb2;
@@ -68,9 +71,30 @@ struct SchemeParams {
   double hyb2, hyb3; // Hybrid_para(2:3) thresholds
 };
 
+// Central-difference coefficients of one axis (OCFD_dxyz_viscous.f90:9-16),
+// formed once on the host with the reference's expressions.
+struct DCoef {
+  double h;
+  double a1, a2, a3;       // CD6: 1/(60h), -3/(20h), 3/(4h)
+  double c1, c2, c3, c4;   // CD8: 0.8/h, -0.2/h, 3.8095...e-2/h, -3.5714...e-3/h
+};
+__host__ inline DCoef make_dcoef(double h) {
+  DCoef c;
+  c.h = h;
+  c.a1 = 1.0 / (60.0 * h);
+  c.a2 = -(3.0 / (20.0 * h));
+  c.a3 = 3.0 / (4.0 * h);
+  c.c1 = 0.8 / h;
+  c.c2 = -(0.2 / h);
+  c.c3 = 3.80952380952380952e-2 / h;
+  c.c4 = -(3.571428571428571428e-3 / h);
+  return c;
+}
+
 struct Phys {
   double gamma, ma, re, pr, cv, cp, dt, ref_amu_t0;
   double h[3];       // hx, hy, hz
+  DCoef dc[3];       // derivative coefficients per axis
   double hinv[3];    // 1/hx ...
   double mass_force[3];
   int if_mass_force;
@@ -90,203 +114,228 @@ template <> struct K<double> {
   static constexpr double ompep = (double)1.e-10f;  // REAL(4) literal (OCFD_Schemes.f90:209)
 };
 
+template <> struct K<float> {
+  static constexpr float w5ep = 1.0e-6f, w5c0 = 3.0f / 10.0f, w5c1 = 3.0f / 5.0f, w5c2 = 1.0f / 10.0f;
+  static constexpr float c1312 = 13.0f / 12.0f, c14 = 1.0f / 4.0f, c13 = 1.0f / 3.0f, c56 = 5.0f / 6.0f,
+                         c16 = 1.0f / 6.0f, c76 = 7.0f / 6.0f, c116 = 11.0f / 6.0f;
+  static constexpr float w7ep = 1.0e-8f;
+  static constexpr float ompep = 1.e-10f;
+};
+
 // WENO5-JS, flux+ (hh_weno5P, OCFD_Schemes_1.f90:9-29; OCFD_weno5P, OCFD_Schemes.f90:6-28)
-__device__ inline double weno5p(const double *x) {
-  typedef K<double> C;
-  const double vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
-  double t, S0, S1, S2;
-  t = v0 - 2.0 * v1 + v2; S0 = C::c1312 * (t * t);
-  t = 3.0 * v0 - 4.0 * v1 + v2; S0 = S0 + C::c14 * (t * t);
-  t = vm1 - 2.0 * v0 + v1; S1 = C::c1312 * (t * t);
+template <typename T>
+__device__ inline T weno5p(const T *x) {
+  typedef K<T> C;
+  const T vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
+  T t, S0, S1, S2;
+  t = v0 - T(2.0) * v1 + v2; S0 = C::c1312 * (t * t);
+  t = T(3.0) * v0 - T(4.0) * v1 + v2; S0 = S0 + C::c14 * (t * t);
+  t = vm1 - T(2.0) * v0 + v1; S1 = C::c1312 * (t * t);
   t = vm1 - v1; S1 = S1 + C::c14 * (t * t);
-  t = vm2 - 2.0 * vm1 + v0; S2 = C::c1312 * (t * t);
-  t = vm2 - 4.0 * vm1 + 3.0 * v0; S2 = S2 + C::c14 * (t * t);
-  double a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-  double a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  double a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
-  double am = a0 + a1 + a2;
-  double q03 = C::c13 * v0 + C::c56 * v1 - C::c16 * v2;
-  double q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
-  double q23 = C::c13 * vm2 - C::c76 * vm1 + C::c116 * v0;
+  t = vm2 - T(2.0) * vm1 + v0; S2 = C::c1312 * (t * t);
+  t = vm2 - T(4.0) * vm1 + T(3.0) * v0; S2 = S2 + C::c14 * (t * t);
+  T a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
+  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
+  T a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
+  T am = a0 + a1 + a2;
+  T q03 = C::c13 * v0 + C::c56 * v1 - C::c16 * v2;
+  T q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
+  T q23 = C::c13 * vm2 - C::c76 * vm1 + C::c116 * v0;
   return (a0 * q03 + a1 * q13 + a2 * q23) / am;
 }
 
 // WENO5-JS, flux- (hh_weno5M, OCFD_Schemes_1.f90:31-52)
-__device__ inline double weno5m(const double *x) {
-  typedef K<double> C;
-  const double vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
-  double t, S0, S1, S2;
-  t = v1 - 2.0 * v0 + vm1; S0 = C::c1312 * (t * t);
-  t = 3.0 * v1 - 4.0 * v0 + vm1; S0 = S0 + C::c14 * (t * t);
-  t = v2 - 2.0 * v1 + v0; S1 = C::c1312 * (t * t);
+template <typename T>
+__device__ inline T weno5m(const T *x) {
+  typedef K<T> C;
+  const T vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
+  T t, S0, S1, S2;
+  t = v1 - T(2.0) * v0 + vm1; S0 = C::c1312 * (t * t);
+  t = T(3.0) * v1 - T(4.0) * v0 + vm1; S0 = S0 + C::c14 * (t * t);
+  t = v2 - T(2.0) * v1 + v0; S1 = C::c1312 * (t * t);
   t = v2 - v0; S1 = S1 + C::c14 * (t * t);
-  t = v3 - 2.0 * v2 + v1; S2 = C::c1312 * (t * t);
-  t = v3 - 4.0 * v2 + 3.0 * v1; S2 = S2 + C::c14 * (t * t);
-  double a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-  double a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  double a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
-  double am = a0 + a1 + a2;
-  double q03 = C::c13 * v1 + C::c56 * v0 - C::c16 * vm1;
-  double q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
-  double q23 = C::c13 * v3 - C::c76 * v2 + C::c116 * v1;
+  t = v3 - T(2.0) * v2 + v1; S2 = C::c1312 * (t * t);
+  t = v3 - T(4.0) * v2 + T(3.0) * v1; S2 = S2 + C::c14 * (t * t);
+  T a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
+  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
+  T a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
+  T am = a0 + a1 + a2;
+  T q03 = C::c13 * v1 + C::c56 * v0 - C::c16 * vm1;
+  T q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
+  T q23 = C::c13 * v3 - C::c76 * v2 + C::c116 * v1;
   return (a0 * q03 + a1 * q13 + a2 * q23) / am;
 }
 
 // WENO5 with the out-of-domain sub-stencil removed (hh_weno5P_boundary,
 // OCFD_Schemes_1.f90:266-295)
-__device__ inline double weno5p_boundary(const double *x, int del) {
-  typedef K<double> C;
-  const double vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
-  double t, S0, S1, S2, a0 = 0.0, q03 = 0.0, a2 = 0.0, q23 = 0.0;
+template <typename T>
+__device__ inline T weno5p_boundary(const T *x, int del) {
+  typedef K<T> C;
+  const T vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
+  T t, S0, S1, S2, a0 = T(0.0), q03 = T(0.0), a2 = T(0.0), q23 = T(0.0);
   if (del != DEL_RIGHT) {
-    t = v0 - 2.0 * v1 + v2; S0 = C::c1312 * (t * t);
-    t = 3.0 * v0 - 4.0 * v1 + v2; S0 = S0 + C::c14 * (t * t);
+    t = v0 - T(2.0) * v1 + v2; S0 = C::c1312 * (t * t);
+    t = T(3.0) * v0 - T(4.0) * v1 + v2; S0 = S0 + C::c14 * (t * t);
     a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
     q03 = C::c13 * v0 + C::c56 * v1 - C::c16 * v2;
   }
-  t = vm1 - 2.0 * v0 + v1; S1 = C::c1312 * (t * t);
+  t = vm1 - T(2.0) * v0 + v1; S1 = C::c1312 * (t * t);
   t = vm1 - v1; S1 = S1 + C::c14 * (t * t);
-  double a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  double q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
+  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
+  T q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
   if (del != DEL_LIFT) {
-    t = vm2 - 2.0 * vm1 + v0; S2 = C::c1312 * (t * t);
-    t = vm2 - 4.0 * vm1 + 3.0 * v0; S2 = S2 + C::c14 * (t * t);
+    t = vm2 - T(2.0) * vm1 + v0; S2 = C::c1312 * (t * t);
+    t = vm2 - T(4.0) * vm1 + T(3.0) * v0; S2 = S2 + C::c14 * (t * t);
     a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
     q23 = C::c13 * vm2 - C::c76 * vm1 + C::c116 * v0;
   }
-  double am = a0 + a1 + a2;
+  T am = a0 + a1 + a2;
   return (a0 * q03 + a1 * q13 + a2 * q23) / am;
 }
 
 // hh_weno5M_boundary (OCFD_Schemes_1.f90:297-325)
-__device__ inline double weno5m_boundary(const double *x, int del) {
-  typedef K<double> C;
-  const double vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
-  double t, S0, S1, S2, a0 = 0.0, q03 = 0.0, a2 = 0.0, q23 = 0.0;
+template <typename T>
+__device__ inline T weno5m_boundary(const T *x, int del) {
+  typedef K<T> C;
+  const T vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
+  T t, S0, S1, S2, a0 = T(0.0), q03 = T(0.0), a2 = T(0.0), q23 = T(0.0);
   if (del != DEL_LIFT) {
-    t = v1 - 2.0 * v0 + vm1; S0 = C::c1312 * (t * t);
-    t = 3.0 * v1 - 4.0 * v0 + vm1; S0 = S0 + C::c14 * (t * t);
+    t = v1 - T(2.0) * v0 + vm1; S0 = C::c1312 * (t * t);
+    t = T(3.0) * v1 - T(4.0) * v0 + vm1; S0 = S0 + C::c14 * (t * t);
     a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
     q03 = C::c13 * v1 + C::c56 * v0 - C::c16 * vm1;
   }
-  t = v2 - 2.0 * v1 + v0; S1 = C::c1312 * (t * t);
+  t = v2 - T(2.0) * v1 + v0; S1 = C::c1312 * (t * t);
   t = v2 - v0; S1 = S1 + C::c14 * (t * t);
-  double a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  double q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
+  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
+  T q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
   if (del != DEL_RIGHT) {
-    t = v3 - 2.0 * v2 + v1; S2 = C::c1312 * (t * t);
-    t = v3 - 4.0 * v2 + 3.0 * v1; S2 = S2 + C::c14 * (t * t);
+    t = v3 - T(2.0) * v2 + v1; S2 = C::c1312 * (t * t);
+    t = v3 - T(4.0) * v2 + T(3.0) * v1; S2 = S2 + C::c14 * (t * t);
     a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
     q23 = C::c13 * v3 - C::c76 * v2 + C::c116 * v1;
   }
-  double am = a0 + a1 + a2;
+  T am = a0 + a1 + a2;
   return (a0 * q03 + a1 * q13 + a2 * q23) / am;
 }
 
 // WENO7-JS core on seven values ordered from the upwind end (OCFD_Schemes.f90:59-193:
 // the P flux uses v(-3..3), the M flux the mirrored v(4..-2)).
-__device__ inline double weno7_core(double m3, double m2, double m1, double c0, double p1, double p2,
-                                    double p3) {
-  constexpr double C0 = 1.0 / 35.0, C1 = 12.0 / 35.0, C2 = 18.0 / 35.0, C3 = 4.0 / 35.0;
-  constexpr double a11 = -2.0 / 6.0, a12 = 9.0 / 6.0, a13 = -18.0 / 6.0, a14 = 11.0 / 6.0;
-  constexpr double a21 = 1.0 / 6.0, a23 = 3.0 / 6.0, a24 = 2.0 / 6.0;
-  constexpr double a31 = -2.0 / 6.0, a32 = -3.0 / 6.0, a34 = -1.0 / 6.0;
-  constexpr double a41 = -11.0 / 6.0, a42 = 18.0 / 6.0, a43 = -9.0 / 6.0, a44 = 2.0 / 6.0;
-  constexpr double b12 = 4.0, b13 = -5.0, b14 = 2.0, b22 = -2.0, b41 = 2.0, b42 = -5.0, b43 = 4.0,
-                   c12 = 3.0;
-  constexpr double d12 = 13.0 / 12.0, d13 = 1043.0 / 960.0, d14 = 1.0 / 12.0;
-  constexpr double e11 = -3.0 / 12.0, e12 = 13.0 / 12.0, e13 = -23.0 / 12.0, e14 = 25.0 / 12.0;
-  constexpr double e21 = 1.0 / 12.0, e22 = -5.0 / 12.0, e23 = 13.0 / 12.0, e24 = 3.0 / 12.0;
-  constexpr double e31 = -1.0 / 12.0, e32 = 7.0 / 12.0, e33 = 7.0 / 12.0, e34 = -1.0 / 12.0;
-  constexpr double e41 = 3.0 / 12.0, e42 = 13.0 / 12.0, e43 = -5.0 / 12.0, e44 = 1.0 / 12.0;
-  constexpr double ep = 1.0e-8;
-  double S10 = a11 * m3 + a12 * m2 + a13 * m1 + a14 * c0;
-  double S11 = a21 * m2 - m1 + a23 * c0 + a24 * p1;
-  double S12 = a31 * m1 + a32 * c0 + p1 + a34 * p2;
-  double S13 = a41 * c0 + a42 * p1 + a43 * p2 + a44 * p3;
-  double S20 = -m3 + b12 * m2 + b13 * m1 + b14 * c0;
-  double S21 = m1 + b22 * c0 + p1;
-  double S22 = c0 + b22 * p1 + p2;
-  double S23 = b41 * c0 + b42 * p1 + b43 * p2 - p3;
-  double S30 = -m3 + c12 * (m2 - m1) + c0;
-  double S31 = -m2 + c12 * (m1 - c0) + p1;
-  double S32 = -m1 + c12 * (c0 - p1) + p2;
-  double S33 = -c0 + c12 * (p1 - p2) + p3;
-  double S0 = S10 * S10 + d12 * S20 * S20 + d13 * S30 * S30 + d14 * S10 * S30;
-  double S1 = S11 * S11 + d12 * S21 * S21 + d13 * S31 * S31 + d14 * S11 * S31;
-  double S2 = S12 * S12 + d12 * S22 * S22 + d13 * S32 * S32 + d14 * S12 * S32;
-  double S3 = S13 * S13 + d12 * S23 * S23 + d13 * S33 * S33 + d14 * S13 * S33;
-  double a0 = C0 / ((ep + S0) * (ep + S0));
-  double a1 = C1 / ((ep + S1) * (ep + S1));
-  double a2 = C2 / ((ep + S2) * (ep + S2));
-  double a3 = C3 / ((ep + S3) * (ep + S3));
-  double am = a0 + a1 + a2 + a3;
-  double q0 = e11 * m3 + e12 * m2 + e13 * m1 + e14 * c0;
-  double q1 = e21 * m2 + e22 * m1 + e23 * c0 + e24 * p1;
-  double q2 = e31 * m1 + e32 * c0 + e33 * p1 + e34 * p2;
-  double q3 = e41 * c0 + e42 * p1 + e43 * p2 + e44 * p3;
+template <typename T>
+__device__ inline T weno7_core(T m3, T m2, T m1, T c0, T p1, T p2,
+                                    T p3) {
+  constexpr T C0 = T(1.0) / T(35.0), C1 = T(12.0) / T(35.0), C2 = T(18.0) / T(35.0), C3 = T(4.0) / T(35.0);
+  constexpr T a11 = -T(2.0) / T(6.0), a12 = T(9.0) / T(6.0), a13 = -T(18.0) / T(6.0), a14 = T(11.0) / T(6.0);
+  constexpr T a21 = T(1.0) / T(6.0), a23 = T(3.0) / T(6.0), a24 = T(2.0) / T(6.0);
+  constexpr T a31 = -T(2.0) / T(6.0), a32 = -T(3.0) / T(6.0), a34 = -T(1.0) / T(6.0);
+  constexpr T a41 = -T(11.0) / T(6.0), a42 = T(18.0) / T(6.0), a43 = -T(9.0) / T(6.0), a44 = T(2.0) / T(6.0);
+  constexpr T b12 = T(4.0), b13 = -T(5.0), b14 = T(2.0), b22 = -T(2.0), b41 = T(2.0), b42 = -T(5.0), b43 = T(4.0),
+                   c12 = T(3.0);
+  constexpr T d12 = T(13.0) / T(12.0), d13 = T(1043.0) / T(960.0), d14 = T(1.0) / T(12.0);
+  constexpr T e11 = -T(3.0) / T(12.0), e12 = T(13.0) / T(12.0), e13 = -T(23.0) / T(12.0), e14 = T(25.0) / T(12.0);
+  constexpr T e21 = T(1.0) / T(12.0), e22 = -T(5.0) / T(12.0), e23 = T(13.0) / T(12.0), e24 = T(3.0) / T(12.0);
+  constexpr T e31 = -T(1.0) / T(12.0), e32 = T(7.0) / T(12.0), e33 = T(7.0) / T(12.0), e34 = -T(1.0) / T(12.0);
+  constexpr T e41 = T(3.0) / T(12.0), e42 = T(13.0) / T(12.0), e43 = -T(5.0) / T(12.0), e44 = T(1.0) / T(12.0);
+  constexpr T ep = T(1.0e-8);
+  T S10 = a11 * m3 + a12 * m2 + a13 * m1 + a14 * c0;
+  T S11 = a21 * m2 - m1 + a23 * c0 + a24 * p1;
+  T S12 = a31 * m1 + a32 * c0 + p1 + a34 * p2;
+  T S13 = a41 * c0 + a42 * p1 + a43 * p2 + a44 * p3;
+  T S20 = -m3 + b12 * m2 + b13 * m1 + b14 * c0;
+  T S21 = m1 + b22 * c0 + p1;
+  T S22 = c0 + b22 * p1 + p2;
+  T S23 = b41 * c0 + b42 * p1 + b43 * p2 - p3;
+  T S30 = -m3 + c12 * (m2 - m1) + c0;
+  T S31 = -m2 + c12 * (m1 - c0) + p1;
+  T S32 = -m1 + c12 * (c0 - p1) + p2;
+  T S33 = -c0 + c12 * (p1 - p2) + p3;
+  T S0 = S10 * S10 + d12 * S20 * S20 + d13 * S30 * S30 + d14 * S10 * S30;
+  T S1 = S11 * S11 + d12 * S21 * S21 + d13 * S31 * S31 + d14 * S11 * S31;
+  T S2 = S12 * S12 + d12 * S22 * S22 + d13 * S32 * S32 + d14 * S12 * S32;
+  T S3 = S13 * S13 + d12 * S23 * S23 + d13 * S33 * S33 + d14 * S13 * S33;
+  T a0 = C0 / ((ep + S0) * (ep + S0));
+  T a1 = C1 / ((ep + S1) * (ep + S1));
+  T a2 = C2 / ((ep + S2) * (ep + S2));
+  T a3 = C3 / ((ep + S3) * (ep + S3));
+  T am = a0 + a1 + a2 + a3;
+  T q0 = e11 * m3 + e12 * m2 + e13 * m1 + e14 * c0;
+  T q1 = e21 * m2 + e22 * m1 + e23 * c0 + e24 * p1;
+  T q2 = e31 * m1 + e32 * c0 + e33 * p1 + e34 * p2;
+  T q3 = e41 * c0 + e42 * p1 + e43 * p2 + e44 * p3;
   return (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / am;
 }
-__device__ inline double weno7p(const double *x) { return weno7_core(x[0], x[1], x[2], x[3], x[4], x[5], x[6]); }
-__device__ inline double weno7m(const double *x) { return weno7_core(x[7], x[6], x[5], x[4], x[3], x[2], x[1]); }
+template <typename T>
+__device__ inline T weno7p(const T *x) { return weno7_core(x[0], x[1], x[2], x[3], x[4], x[5], x[6]); }
+template <typename T>
+__device__ inline T weno7m(const T *x) { return weno7_core(x[7], x[6], x[5], x[4], x[3], x[2], x[1]); }
 
 // Fortran sign(1.d0, x)
-__device__ inline double sgn1(double x) { return copysign(1.0, x); }
-__device__ inline double dmin2(double a, double b) { return a < b ? a : b; }
-__device__ inline double dmax2(double a, double b) { return a > b ? a : b; }
+template <typename T>
+__device__ inline T sgn1(T x) { return copysign(T(1.0), x); }
+template <typename T>
+__device__ inline T dmin2(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ inline T dmax2(T a, T b) { return a > b ? a : b; }
 
 // minmod2 / minmod4 (OCFD_Schemes.f90:274-289)
-__device__ inline double minmod2(double x1, double x2) {
-  return 0.5 * (sgn1(x1) + sgn1(x2)) * dmin2(fabs(x1), fabs(x2));
+template <typename T>
+__device__ inline T minmod2(T x1, T x2) {
+  return T(0.5) * (sgn1(x1) + sgn1(x2)) * dmin2(fabs(x1), fabs(x2));
 }
-__device__ inline double minmod4(double x1, double x2, double x3, double x4) {
-  double r = 0.5 * (sgn1(x1) + sgn1(x2));
-  r = r * fabs(0.5 * (sgn1(x1) + sgn1(x3)));
-  r = r * fabs(0.5 * (sgn1(x1) + sgn1(x4)));
+template <typename T>
+__device__ inline T minmod4(T x1, T x2, T x3, T x4) {
+  T r = T(0.5) * (sgn1(x1) + sgn1(x2));
+  r = r * fabs(T(0.5) * (sgn1(x1) + sgn1(x3)));
+  r = r * fabs(T(0.5) * (sgn1(x1) + sgn1(x4)));
   return r * dmin2(dmin2(dmin2(fabs(x1), fabs(x2)), fabs(x3)), fabs(x4));
 }
 
 // OMP6 with the MP limiter on eight values ordered from the upwind end
 // (hh_OMP6P OCFD_Schemes_1.f90:195-226, hh_OMP6M :228-260).  The reference's M
 // branch uses v(1)-v(0) (x0-x1 in upwind order) in lc where P uses x0-xm1.
-__device__ inline double omp6_core(double xm3, double xm2, double xm1, double x0, double x1, double x2,
-                                   double x3, double x4, bool mflux) {
-  constexpr double om = 0.015, on = 0.0;
-  constexpr double a = 0.5 * (om + on), b = 0.5 * (om - on);
-  constexpr double a1 = a, a2 = 1.0 / 60.0 - b - 6.0 * a, a3 = -(2.0 / 15.0) + 6.0 * b + 15.0 * a;
-  constexpr double a4 = 37.0 / 60.0 - 15.0 * b - 20.0 * a, a5 = 37.0 / 60.0 + 20.0 * b + 15.0 * a;
-  constexpr double a6 = -(2.0 / 15.0) - 6.0 * a - 15.0 * b, a7 = 1.0 / 60.0 + a + 6.0 * b, a8 = -b;
-  constexpr double kappa = 4.0;
-  double mid = a1 * x4 + a2 * x3 + a3 * x2 + a4 * x1 + a5 * x0 + a6 * xm1 + a7 * xm2 + a8 * xm3;
-  double mp = x0 + minmod2((x1 - x0), kappa * (x0 - xm1));
-  if ((mid - x0) * (mid - mp) >= K<double>::ompep) {
-    double d1 = xm2 + x0 - 2.0 * xm1;
-    double d2 = xm1 + x1 - 2.0 * x0;
-    double d3 = x0 + x2 - 2.0 * x1;
-    double ul = x0 + kappa * (x0 - xm1);
-    double md = 0.5 * (x0 + x1) - 0.5 * minmod4(4.0 * d2 - d3, 4.0 * d3 - d2, d2, d3);
-    double lc = x0 + 0.5 * (x0 - (mflux ? x1 : xm1)) + kappa * minmod4(4.0 * d1 - d2, 4.0 * d2 - d1, d2, d1) / 3.0;
-    double fmin = dmax2(dmin2(dmin2(x0, x1), md), dmin2(dmin2(x0, ul), lc));
-    double fmax = dmin2(dmax2(dmax2(x0, x1), md), dmax2(dmax2(x0, ul), lc));
+template <typename T>
+__device__ inline T omp6_core(T xm3, T xm2, T xm1, T x0, T x1, T x2,
+                                   T x3, T x4, bool mflux) {
+  constexpr T om = T(0.015), on = T(0.0);
+  constexpr T a = T(0.5) * (om + on), b = T(0.5) * (om - on);
+  constexpr T a1 = a, a2 = T(1.0) / T(60.0) - b - T(6.0) * a, a3 = -(T(2.0) / T(15.0)) + T(6.0) * b + T(15.0) * a;
+  constexpr T a4 = T(37.0) / T(60.0) - T(15.0) * b - T(20.0) * a, a5 = T(37.0) / T(60.0) + T(20.0) * b + T(15.0) * a;
+  constexpr T a6 = -(T(2.0) / T(15.0)) - T(6.0) * a - T(15.0) * b, a7 = T(1.0) / T(60.0) + a + T(6.0) * b, a8 = -b;
+  constexpr T kappa = T(4.0);
+  T mid = a1 * x4 + a2 * x3 + a3 * x2 + a4 * x1 + a5 * x0 + a6 * xm1 + a7 * xm2 + a8 * xm3;
+  T mp = x0 + minmod2((x1 - x0), kappa * (x0 - xm1));
+  if ((mid - x0) * (mid - mp) >= K<T>::ompep) {
+    T d1 = xm2 + x0 - T(2.0) * xm1;
+    T d2 = xm1 + x1 - T(2.0) * x0;
+    T d3 = x0 + x2 - T(2.0) * x1;
+    T ul = x0 + kappa * (x0 - xm1);
+    T md = T(0.5) * (x0 + x1) - T(0.5) * minmod4(T(4.0) * d2 - d3, T(4.0) * d3 - d2, d2, d3);
+    T lc = x0 + T(0.5) * (x0 - (mflux ? x1 : xm1)) + kappa * minmod4(T(4.0) * d1 - d2, T(4.0) * d2 - d1, d2, d1) / T(3.0);
+    T fmin = dmax2(dmin2(dmin2(x0, x1), md), dmin2(dmin2(x0, ul), lc));
+    T fmax = dmin2(dmax2(dmax2(x0, x1), md), dmax2(dmax2(x0, ul), lc));
     mid = mid + minmod2(fmax - mid, fmin - mid);
   }
   return mid;
 }
-__device__ inline double omp6p(const double *x) {
+template <typename T>
+__device__ inline T omp6p(const T *x) {
   return omp6_core(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], false);
 }
-__device__ inline double omp6m(const double *x) {
+template <typename T>
+__device__ inline T omp6m(const T *x) {
   return omp6_core(x[7], x[6], x[5], x[4], x[3], x[2], x[1], x[0], true);
 }
 
 // UD7L (OCFD_Schemes.f90:292-316)
-__device__ inline double ud7lp(const double *c, const double *x) {
-  return c[0] * x[0] + c[1] * x[1] + c[2] * x[2] + c[3] * x[3] + c[4] * x[4] + c[5] * x[5] + c[6] * x[6] +
-         c[7] * x[7];
+template <typename T>
+__device__ inline T ud7lp(const double *c, const T *x) {
+  return T(c[0]) * x[0] + T(c[1]) * x[1] + T(c[2]) * x[2] + T(c[3]) * x[3] + T(c[4]) * x[4] + T(c[5]) * x[5] +
+         T(c[6]) * x[6] + T(c[7]) * x[7];
 }
-__device__ inline double ud7lm(const double *c, const double *x) {
-  return c[0] * x[7] + c[1] * x[6] + c[2] * x[5] + c[3] * x[4] + c[4] * x[3] + c[5] * x[2] + c[6] * x[1] +
-         c[7] * x[0];
+template <typename T>
+__device__ inline T ud7lm(const double *c, const T *x) {
+  return T(c[0]) * x[7] + T(c[1]) * x[6] + T(c[2]) * x[5] + T(c[3]) * x[4] + T(c[4]) * x[3] + T(c[5]) * x[2] +
+         T(c[6]) * x[1] + T(c[7]) * x[0];
 }
 
 // set_hybrid_scheme (OCFD_SetHybrid.f90:207-220)
@@ -296,8 +345,8 @@ __device__ inline int hybrid_index(double r0, double t2, double t3) {
 
 // Inner-scheme flux of one face (the scheme select of OCFD2d_flux1/2 and
 // flux_x1/x2; OCFD_HybridP/M for the hybrid scheme).
-template <int SCH>
-__device__ inline double inner_p(const SchemeParams &S, const double *x, int scm) {
+template <int SCH, typename T>
+__device__ inline T inner_p(const SchemeParams &S, const T *x, int scm) {
   if (SCH == SCH_WENO5) return weno5p(x);
   if (SCH == SCH_WENO7) return weno7p(x);
   if (SCH == SCH_OMP6) return omp6p(x);
@@ -306,8 +355,8 @@ __device__ inline double inner_p(const SchemeParams &S, const double *x, int scm
   if (scm == 2) return weno7p(x);
   return weno5p(x);
 }
-template <int SCH>
-__device__ inline double inner_m(const SchemeParams &S, const double *x, int scm) {
+template <int SCH, typename T>
+__device__ inline T inner_m(const SchemeParams &S, const T *x, int scm) {
   if (SCH == SCH_WENO5) return weno5m(x);
   if (SCH == SCH_WENO7) return weno7m(x);
   if (SCH == SCH_OMP6) return omp6m(x);
@@ -319,14 +368,14 @@ __device__ inline double inner_m(const SchemeParams &S, const double *x, int scm
 
 // Flux+ of face s (left cell s, 0 <= s <= n) including the block-edge closures
 // of OCFD2d_flux1 (OCFD_inviscous.f90:210-283).  x[k] = f(s+k-3).
-template <int SCH>
-__device__ inline double face_p_nonchar(const SchemeParams &S, const double *x, int s, int n, int axis,
+template <int SCH, typename T>
+__device__ inline T face_p_nonchar(const SchemeParams &S, const T *x, int s, int n, int axis,
                                         int scm) {
   const int lo = S.bidx[axis][0], hi = S.bidx[axis][1];
   const int sl = S.sb[2 * axis], sr = S.sb[2 * axis + 1];
   if (lo == 1 && s <= -S.ka1) {
     if (sl == 0) {
-      if (s == 0 || s == 1) return (2.0 * x[4 - s] + 5.0 * x[5 - s] - x[6 - s]) / 6.0;
+      if (s == 0 || s == 1) return (T(2.0) * x[4 - s] + T(5.0) * x[5 - s] - x[6 - s]) / T(6.0);
       if (s == 2) return weno5p_boundary(x, DEL_LIFT);
       return weno5p(x);
     }
@@ -334,7 +383,7 @@ __device__ inline double face_p_nonchar(const SchemeParams &S, const double *x, 
   }
   if (hi == 1 && s >= n - S.kb1 + 1) {
     if (sr == 0) {
-      if (s == n) return (2.0 * x[1] - 7.0 * x[2] + 11.0 * x[3]) / 6.0;
+      if (s == n) return (T(2.0) * x[1] - T(7.0) * x[2] + T(11.0) * x[3]) / T(6.0);
       if (s == n - 1) return weno5p_boundary(x, DEL_RIGHT);
       return weno5p(x);
     }
@@ -344,14 +393,14 @@ __device__ inline double face_p_nonchar(const SchemeParams &S, const double *x, 
 }
 
 // Flux- of face s with the closures of OCFD2d_flux2 (OCFD_inviscous.f90:287-361).
-template <int SCH>
-__device__ inline double face_m_nonchar(const SchemeParams &S, const double *x, int s, int n, int axis,
+template <int SCH, typename T>
+__device__ inline T face_m_nonchar(const SchemeParams &S, const T *x, int s, int n, int axis,
                                         int scm) {
   const int lo = S.bidx[axis][0], hi = S.bidx[axis][1];
   const int sl = S.sb[2 * axis], sr = S.sb[2 * axis + 1];
   if (lo == 1 && s <= -S.ka2) {
     if (sl == 0) {
-      if (s == 0) return (2.0 * x[6] - 7.0 * x[5] + 11.0 * x[4]) / 6.0;
+      if (s == 0) return (T(2.0) * x[6] - T(7.0) * x[5] + T(11.0) * x[4]) / T(6.0);
       if (s == 1) return weno5m_boundary(x, DEL_LIFT);
       return weno5m(x);
     }
@@ -359,8 +408,8 @@ __device__ inline double face_m_nonchar(const SchemeParams &S, const double *x, 
   }
   if (hi == 1 && s >= n - S.kb2 + 1) {
     if (sr == 0) {
-      if (s == n) return (2.0 * x[3] + 5.0 * x[2] - x[1]) / 6.0;
-      if (s == n - 1) return (2.0 * x[4] + 5.0 * x[3] - x[2]) / 6.0;
+      if (s == n) return (T(2.0) * x[3] + T(5.0) * x[2] - x[1]) / T(6.0);
+      if (s == n - 1) return (T(2.0) * x[4] + T(5.0) * x[3] - x[2]) / T(6.0);
       if (s == n - 2) return weno5m_boundary(x, DEL_RIGHT);
       return weno5m(x);
     }
@@ -371,15 +420,15 @@ __device__ inline double face_m_nonchar(const SchemeParams &S, const double *x, 
 
 // flux_x1 (OCFD_inviscous_character.f90:313-364) for face s of the
 // characteristic path; x[k] = projected value at s+k-3.
-template <int SCH>
-__device__ inline double face_p_char(const SchemeParams &S, const double *x, int s, int n, int axis, int scm) {
+template <int SCH, typename T>
+__device__ inline T face_p_char(const SchemeParams &S, const T *x, int s, int n, int axis, int scm) {
   const int lo = S.bidx[axis][0], hi = S.bidx[axis][1];
   const int sl = S.sb[2 * axis], sr = S.sb[2 * axis + 1];
   const int Ka = S.ka1, Kb = S.kb1;
   if ((lo == 0 || s > -Ka || sl == -1) && (hi == 0 || s <= n - Kb || sr == -1)) return inner_p<SCH>(S, x, scm);
   if (s <= -Ka && lo == 1 && sl != -1) {
     if (sl == 0) {
-      if (s == 1) return (2.0 * x[3] + 5.0 * x[4] - x[5]) / 6.0;
+      if (s == 1) return (T(2.0) * x[3] + T(5.0) * x[4] - x[5]) / T(6.0);
       if (s == 2) return weno5p_boundary(x, DEL_LIFT);
       return weno5p(x);
     }
@@ -393,8 +442,8 @@ __device__ inline double face_p_char(const SchemeParams &S, const double *x, int
 }
 
 // flux_x2 (OCFD_inviscous_character.f90:367-416)
-template <int SCH>
-__device__ inline double face_m_char(const SchemeParams &S, const double *x, int s, int n, int axis, int scm) {
+template <int SCH, typename T>
+__device__ inline T face_m_char(const SchemeParams &S, const T *x, int s, int n, int axis, int scm) {
   const int lo = S.bidx[axis][0], hi = S.bidx[axis][1];
   const int sl = S.sb[2 * axis], sr = S.sb[2 * axis + 1];
   const int Ka = S.ka2, Kb = S.kb2;
@@ -407,7 +456,7 @@ __device__ inline double face_m_char(const SchemeParams &S, const double *x, int
     return weno5m(x);
   }
   if (sr == 0) {
-    if (s == n - 1) return (2.0 * x[4] + 5.0 * x[3] - x[2]) / 6.0;
+    if (s == n - 1) return (T(2.0) * x[4] + T(5.0) * x[3] - x[2]) / T(6.0);
     if (s == n - 2) return weno5m_boundary(x, DEL_RIGHT);
     return weno5m(x);
   }
@@ -418,25 +467,26 @@ __device__ inline double face_m_char(const SchemeParams &S, const double *x, int
 // flux-vector splitting of one point (OCFD_flux_split.f90)
 // ---------------------------------------------------------------------------
 // split_Stager_Warming (OCFD_flux_split.f90:5-70)
-__device__ inline void split_sw(double gamma, double c1, double d1, double u1, double v1, double w1, double A1,
-                                double A2, double A3, double *fp, double *fm) {
-  const double tmp1 = 2.0 * (gamma - 1.0);
-  const double tmp3 = (3.0 - gamma) / (2.0 * (gamma - 1.0));
-  double ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
-  double ss1 = 1.0 / ss;
-  double ak1 = A1 * ss1, ak2 = A2 * ss1, ak3 = A3 * ss1;
-  double vs = A1 * u1 + A2 * v1 + A3 * w1;
-  double E1 = vs, E2 = vs - c1 * ss, E3 = vs + c1 * ss;
-  double E1P = (E1 + fabs(E1)) * 0.5, E2P = (E2 + fabs(E2)) * 0.5, E3P = (E3 + fabs(E3)) * 0.5;
-  double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
-  double tmp0 = d1 / (2.0 * gamma);
-  double uc1 = u1 - c1 * ak1, uc2 = u1 + c1 * ak1;
-  double vc1 = v1 - c1 * ak2, vc2 = v1 + c1 * ak2;
-  double wc1 = w1 - c1 * ak3, wc2 = w1 + c1 * ak3;
-  double vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * 0.5;
-  double vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * 0.5;
-  double vv = (gamma - 1.0) * (u1 * u1 + v1 * v1 + w1 * w1);
-  double W2 = tmp3 * c1 * c1;
+template <typename T>
+__device__ inline void split_sw(T gamma, T c1, T d1, T u1, T v1, T w1, T A1,
+                                T A2, T A3, T *fp, T *fm) {
+  const T tmp1 = T(2.0) * (gamma - T(1.0));
+  const T tmp3 = (T(3.0) - gamma) / (T(2.0) * (gamma - T(1.0)));
+  T ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+  T ss1 = T(1.0) / ss;
+  T ak1 = A1 * ss1, ak2 = A2 * ss1, ak3 = A3 * ss1;
+  T vs = A1 * u1 + A2 * v1 + A3 * w1;
+  T E1 = vs, E2 = vs - c1 * ss, E3 = vs + c1 * ss;
+  T E1P = (E1 + fabs(E1)) * T(0.5), E2P = (E2 + fabs(E2)) * T(0.5), E3P = (E3 + fabs(E3)) * T(0.5);
+  T E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+  T tmp0 = d1 / (T(2.0) * gamma);
+  T uc1 = u1 - c1 * ak1, uc2 = u1 + c1 * ak1;
+  T vc1 = v1 - c1 * ak2, vc2 = v1 + c1 * ak2;
+  T wc1 = w1 - c1 * ak3, wc2 = w1 + c1 * ak3;
+  T vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * T(0.5);
+  T vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * T(0.5);
+  T vv = (gamma - T(1.0)) * (u1 * u1 + v1 * v1 + w1 * w1);
+  T W2 = tmp3 * c1 * c1;
   fp[0] = tmp0 * (tmp1 * E1P + E2P + E3P);
   fp[1] = tmp0 * (tmp1 * E1P * u1 + E2P * uc1 + E3P * uc2);
   fp[2] = tmp0 * (tmp1 * E1P * v1 + E2P * vc1 + E3P * vc2);
@@ -450,42 +500,46 @@ __device__ inline void split_sw(double gamma, double c1, double d1, double u1, d
 }
 
 // split_Local_LaxFriedrichs (OCFD_flux_split.f90:73-104)
-__device__ inline void split_llf(double gamma, double c1, double d1, double u1, double v1, double w1,
-                                 double A1, double A2, double A3, double *fp, double *fm) {
-  const double tmp0 = 1.0 / gamma, tmp1 = 1.0 / (gamma - 1.0);
-  double ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
-  double un = A1 * u1 + A2 * v1 + A3 * w1;
-  double lamda0 = fabs(un) + ss * c1;
-  double p1 = tmp0 * d1 * c1 * c1;
-  double E0 = tmp1 * p1 + 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
-  double h0 = E0 + p1;
-  fp[0] = 0.5 * (d1 * un + lamda0 * d1);
-  fp[1] = 0.5 * (d1 * u1 * un + A1 * p1 + lamda0 * d1 * u1);
-  fp[2] = 0.5 * (d1 * v1 * un + A2 * p1 + lamda0 * d1 * v1);
-  fp[3] = 0.5 * (d1 * w1 * un + A3 * p1 + lamda0 * d1 * w1);
-  fp[4] = 0.5 * (h0 * un + lamda0 * E0);
-  fm[0] = 0.5 * (d1 * un - lamda0 * d1);
-  fm[1] = 0.5 * (d1 * u1 * un + A1 * p1 - lamda0 * d1 * u1);
-  fm[2] = 0.5 * (d1 * v1 * un + A2 * p1 - lamda0 * d1 * v1);
-  fm[3] = 0.5 * (d1 * w1 * un + A3 * p1 - lamda0 * d1 * w1);
-  fm[4] = 0.5 * (h0 * un - lamda0 * E0);
+template <typename T>
+__device__ inline void split_llf(T gamma, T c1, T d1, T u1, T v1, T w1,
+                                 T A1, T A2, T A3, T *fp, T *fm) {
+  const T tmp0 = T(1.0) / gamma, tmp1 = T(1.0) / (gamma - T(1.0));
+  T ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+  T un = A1 * u1 + A2 * v1 + A3 * w1;
+  T lamda0 = fabs(un) + ss * c1;
+  T p1 = tmp0 * d1 * c1 * c1;
+  T E0 = tmp1 * p1 + T(0.5) * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
+  T h0 = E0 + p1;
+  fp[0] = T(0.5) * (d1 * un + lamda0 * d1);
+  fp[1] = T(0.5) * (d1 * u1 * un + A1 * p1 + lamda0 * d1 * u1);
+  fp[2] = T(0.5) * (d1 * v1 * un + A2 * p1 + lamda0 * d1 * v1);
+  fp[3] = T(0.5) * (d1 * w1 * un + A3 * p1 + lamda0 * d1 * w1);
+  fp[4] = T(0.5) * (h0 * un + lamda0 * E0);
+  fm[0] = T(0.5) * (d1 * un - lamda0 * d1);
+  fm[1] = T(0.5) * (d1 * u1 * un + A1 * p1 - lamda0 * d1 * u1);
+  fm[2] = T(0.5) * (d1 * v1 * un + A2 * p1 - lamda0 * d1 * v1);
+  fm[3] = T(0.5) * (d1 * w1 * un + A3 * p1 - lamda0 * d1 * w1);
+  fm[4] = T(0.5) * (h0 * un - lamda0 * E0);
 }
 
 // Left (S) / right (S1) eigenvector matrices of the face-averaged state
 // (OCFD_inviscous_character.f90:205-248).
-struct CharBasis {
-  double u1, v1, w1, c1, n1, n2, n3, l1, l2, l3, m1, m2, m3, un, ul, um, V2, KK, KK1, X1, H;
+template <typename T>
+struct CharBasisT {
+  T u1, v1, w1, c1, n1, n2, n3, l1, l2, l3, m1, m2, m3, un, ul, um, V2, KK, KK1, X1, H;
 };
-__device__ inline void char_basis(double gamma, double u1, double v1, double w1, double c1, double a1, double a2,
-                                  double a3, CharBasis &B) {
-  double ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+typedef CharBasisT<double> CharBasis;
+template <typename T>
+__device__ inline void char_basis(T gamma, T u1, T v1, T w1, T c1, T a1, T a2,
+                                  T a3, CharBasisT<T> &B) {
+  T ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
   B.n1 = a1 / ss; B.n2 = a2 / ss; B.n3 = a3 / ss;
   if (fabs(B.n3) <= fabs(B.n2)) {
     ss = sqrt(B.n1 * B.n1 + B.n2 * B.n2);
-    B.l1 = -B.n2 / ss; B.l2 = B.n1 / ss; B.l3 = 0.0;
+    B.l1 = -B.n2 / ss; B.l2 = B.n1 / ss; B.l3 = T(0.0);
   } else {
     ss = sqrt(B.n1 * B.n1 + B.n3 * B.n3);
-    B.l1 = -B.n3 / ss; B.l2 = 0.0; B.l3 = B.n1 / ss;
+    B.l1 = -B.n3 / ss; B.l2 = T(0.0); B.l3 = B.n1 / ss;
   }
   B.m1 = B.n2 * B.l3 - B.n3 * B.l2;
   B.m2 = B.n3 * B.l1 - B.n1 * B.l3;
@@ -493,19 +547,20 @@ __device__ inline void char_basis(double gamma, double u1, double v1, double w1,
   B.un = u1 * B.n1 + v1 * B.n2 + w1 * B.n3;
   B.ul = u1 * B.l1 + v1 * B.l2 + w1 * B.l3;
   B.um = u1 * B.m1 + v1 * B.m2 + w1 * B.m3;
-  B.V2 = (u1 * u1 + v1 * v1 + w1 * w1) * 0.5;
-  B.KK = (gamma - 1.0) / (c1 * c1);
-  B.KK1 = B.KK * 0.5;
-  B.X1 = 1.0 / (2.0 * c1);
-  B.H = B.V2 + 1.0 / B.KK;
+  B.V2 = (u1 * u1 + v1 * v1 + w1 * w1) * T(0.5);
+  B.KK = (gamma - T(1.0)) / (c1 * c1);
+  B.KK1 = B.KK * T(0.5);
+  B.X1 = T(1.0) / (T(2.0) * c1);
+  B.H = B.V2 + T(1.0) / B.KK;
   B.u1 = u1; B.v1 = v1; B.w1 = w1; B.c1 = c1;
 }
 // row r of S (OCFD_inviscous_character.f90:238-242)
-__device__ inline void char_S_row(const CharBasis &B, int r, double *s) {
+template <typename T>
+__device__ inline void char_S_row(const CharBasisT<T> &B, int r, T *s) {
   switch (r) {
-    case 0: s[0] = 1.0 - B.KK * B.V2; s[1] = B.KK * B.u1; s[2] = B.KK * B.v1; s[3] = B.KK * B.w1; s[4] = -B.KK; break;
-    case 1: s[0] = -B.ul; s[1] = B.l1; s[2] = B.l2; s[3] = B.l3; s[4] = 0.0; break;
-    case 2: s[0] = -B.um; s[1] = B.m1; s[2] = B.m2; s[3] = B.m3; s[4] = 0.0; break;
+    case 0: s[0] = T(1.0) - B.KK * B.V2; s[1] = B.KK * B.u1; s[2] = B.KK * B.v1; s[3] = B.KK * B.w1; s[4] = -B.KK; break;
+    case 1: s[0] = -B.ul; s[1] = B.l1; s[2] = B.l2; s[3] = B.l3; s[4] = T(0.0); break;
+    case 2: s[0] = -B.um; s[1] = B.m1; s[2] = B.m2; s[3] = B.m3; s[4] = T(0.0); break;
     case 3:
       s[0] = B.KK1 * B.V2 + B.X1 * B.un; s[1] = -(B.X1 * B.n1) - B.KK1 * B.u1;
       s[2] = -(B.X1 * B.n2) - B.KK1 * B.v1; s[3] = -(B.X1 * B.n3) - B.KK1 * B.w1; s[4] = B.KK1; break;
@@ -515,9 +570,10 @@ __device__ inline void char_S_row(const CharBasis &B, int r, double *s) {
   }
 }
 // hh = S1 * h0 (OCFD_inviscous_character.f90:244-248, 284-288)
-__device__ inline void char_back(const CharBasis &B, const double *h0, double *hh) {
-  const double c1 = B.c1;
-  hh[0] = 1.0 * h0[0] + 0.0 * h0[1] + 0.0 * h0[2] + 1.0 * h0[3] + 1.0 * h0[4];
+template <typename T>
+__device__ inline void char_back(const CharBasisT<T> &B, const T *h0, T *hh) {
+  const T c1 = B.c1;
+  hh[0] = T(1.0) * h0[0] + T(0.0) * h0[1] + T(0.0) * h0[2] + T(1.0) * h0[3] + T(1.0) * h0[4];
   hh[1] = B.u1 * h0[0] + B.l1 * h0[1] + B.m1 * h0[2] + (B.u1 - c1 * B.n1) * h0[3] + (B.u1 + c1 * B.n1) * h0[4];
   hh[2] = B.v1 * h0[0] + B.l2 * h0[1] + B.m2 * h0[2] + (B.v1 - c1 * B.n2) * h0[3] + (B.v1 + c1 * B.n2) * h0[4];
   hh[3] = B.w1 * h0[0] + B.l3 * h0[1] + B.m3 * h0[2] + (B.w1 - c1 * B.n3) * h0[3] + (B.w1 + c1 * B.n3) * h0[4];
@@ -532,29 +588,21 @@ __device__ inline void char_back(const CharBasis &B, const double *h0, double *h
 // non-periodic axes only.
 // ---------------------------------------------------------------------------
 template <typename F>
-__device__ inline double dxyz0_point(F o, int s, int n, double h, bool cd8, bool periodic) {
+__device__ inline auto dxyz0_point(F o, int s, int n, const DCoef &c, bool cd8, bool periodic) -> decltype(o(0)) {
+  typedef decltype(o(0)) T;
+  const T h = T(c.h), a1 = T(c.a1), a2 = T(c.a2), a3 = T(c.a3);
   if (!periodic) {
-    if (s == 1) return (-(3.0 * o(0)) + 4.0 * o(1) - o(2)) / (2.0 * h);
-    if (s == 2) return (o(1) - o(-1)) / (2.0 * h);
-    if (s == 3) return (8.0 * (o(1) - o(-1)) - (o(2) - o(-2))) / (12.0 * h);
-    if (cd8 && s == 4) {
-      const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-      return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
-    }
-    if (s == n) return (o(-2) - 4.0 * o(-1) + 3.0 * o(0)) / (2.0 * h);
-    if (s == n - 1) return (o(1) - o(-1)) / (2.0 * h);
-    if (s == n - 2) return (8.0 * (o(1) - o(-1)) - (o(2) - o(-2))) / (12.0 * h);
-    if (cd8 && s == n - 3) {
-      const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-      return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
-    }
+    if (s == 1) return (-(T(3.0) * o(0)) + T(4.0) * o(1) - o(2)) / (T(2.0) * h);
+    if (s == 2) return (o(1) - o(-1)) / (T(2.0) * h);
+    if (s == 3) return (T(8.0) * (o(1) - o(-1)) - (o(2) - o(-2))) / (T(12.0) * h);
+    if (cd8 && s == 4) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
+    if (s == n) return (o(-2) - T(4.0) * o(-1) + T(3.0) * o(0)) / (T(2.0) * h);
+    if (s == n - 1) return (o(1) - o(-1)) / (T(2.0) * h);
+    if (s == n - 2) return (T(8.0) * (o(1) - o(-1)) - (o(2) - o(-2))) / (T(12.0) * h);
+    if (cd8 && s == n - 3) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
   }
-  if (!cd8) {
-    const double a1 = 1.0 / (60.0 * h), a2 = -(3.0 / (20.0 * h)), a3 = 3.0 / (4.0 * h);
-    return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
-  }
-  const double c1 = 0.8 / h, c2 = -(0.2 / h), c3 = 3.80952380952380952e-2 / h, c4 = -(3.571428571428571428e-3 / h);
-  return c1 * (o(1) - o(-1)) + c2 * (o(2) - o(-2)) + c3 * (o(3) - o(-3)) + c4 * (o(4) - o(-4));
+  if (!cd8) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
+  return T(c.c1) * (o(1) - o(-1)) + T(c.c2) * (o(2) - o(-2)) + T(c.c3) * (o(3) - o(-3)) + T(c.c4) * (o(4) - o(-4));
 }
 
 }  // namespace ocfd

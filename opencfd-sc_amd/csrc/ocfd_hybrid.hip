@@ -1,0 +1,242 @@
+// ocfd_hybrid.hip -- the nonlinear faces of the hybrid scheme, gathered.
+//
+// The hybrid scheme (OCFD_Schemes.f90:319-358, OCFD_SetHybrid.f90:207-220)
+// reconstructs most faces with the linear UD7L blend and a few with WENO7 or
+// WENO5, chosen per face from the sensor Rhybrid that comput_Rhybrid forms
+// once per time step.  Computed inside the sweep tiles, the few WENO faces
+// leave most lanes of a tile idle; here they are
+//   k_hyb_list    listed once per step, per sweep axis (wave-aggregated
+//                 appends), with a face -> slot map for the sweeps;
+//   k_weno_faces  computed densely before each stage's sweeps: batches of
+//                 listed faces per workgroup, split fluxes of the 8 stencil
+//                 points in LDS, then one (face, characteristic row) -- or
+//                 (face, component) without projection -- per lane.
+// The sweep kernels read the listed faces' fluxes by slot.  Every flux is the
+// same expression as in the sweep (flux_charteric,
+// OCFD_inviscous_character.f90:190-290; OCFD2d_flux1/2,
+// OCFD_inviscous.f90:210-361), so results are unchanged bit for bit.
+#include "ocfd_kernels.h"
+
+namespace ocfd {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int FB = 48;  // listed faces per workgroup batch
+
+// One thread per face of the axis: scheme index from the sensor, append
+// the WENO faces to the axis list, write the slot map.
+template <int AXIS>
+__global__ __launch_bounds__(NT) void k_hyb_list(const Block B) {
+  typedef TileIndex<AXIS> TI;
+  const Geom &g = B.g;
+  const SchemeParams &S = B.S;
+  const int n = g.n(AXIS);
+  const int64_t nfa = TI::nfaces(g);
+  const int64_t fid = (int64_t)blockIdx.x * NT + threadIdx.x;
+  bool weno = false;
+  int code = 0;
+  if (fid < nfa) {
+    int s, w, c;
+    TI::face_of(g, fid, s, w, c);
+    // set_hybrid_scheme: the larger sensor value of the two cells of the face
+    const double r0 = dmax2(B.rhyb[TI::hidx(g, s, w, c)], B.rhyb[TI::hidx(g, s + 1, w, c)]);
+    const int scm = hybrid_index(r0, S.hyb2, S.hyb3);
+    // the characteristic path closes non-periodic block edges with the
+    // non-reflecting face whatever the scheme index
+    const bool closure = S.is_char && ((S.bidx[AXIS][0] == 1 && S.sb[2 * AXIS] == 0 && s == 0) ||
+                                       (S.bidx[AXIS][1] == 1 && S.sb[2 * AXIS + 1] == 0 && s == n));
+    weno = scm >= 2 && !closure;
+    code = (int)(fid * 2 + (scm == 3 ? 1 : 0));
+  }
+  const unsigned long long mask = __ballot(weno);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    if (lane == leader) base = atomicAdd(&B.hcount[AXIS], __popcll(mask));
+    base = __shfl(base, leader);
+  }
+  const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+  if (fid < nfa) B.hmap[AXIS][fid] = weno ? slot : -1;
+  if (weno) B.hlist[AXIS][slot] = code;
+}
+
+// Fluxes of the listed faces of one axis.  A fixed grid strides over batches
+// of FB faces; the list length is read from the device counter.
+template <int AXIS, bool CHAR, bool FAST>
+__global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
+  typedef TileIndex<AXIS> TI;
+  constexpr int NH = CHAR ? 5 : 10;
+  __shared__ double sF[FB][8][10];  // fp(1:5), fm(1:5) at stencil points s-3 .. s+4
+  __shared__ double sQ[FB][2][7];   // u, v, w, c, A1..A3 at points s, s+1
+  __shared__ double sB[FB][21];     // characteristic basis of the face
+  __shared__ double sG[FB][5];      // characteristic fluxes h0(1:5)
+  const Geom &g = B.g;
+  const SchemeParams &S = B.S;
+  const int n = g.n(AXIS);
+  const int tid = threadIdx.x;
+  const double gamma = B.P.gamma, rma = B.P.ma;
+  const int q1 = M_AKX1 + 3 * AXIS;
+  const int count = B.hcount[AXIS];
+  const int nbatch = (count + FB - 1) / FB;
+  const int *list = B.hlist[AXIS];
+  double *out = B.wflux[AXIS];
+  for (int batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+    const int f0 = batch * FB;
+    const int nf = count - f0 < FB ? count - f0 : FB;
+    // split fluxes at the stencil points (OCFD_inviscous.f90:64-70, split_*)
+    for (int it = tid; it < nf * 8; it += NT) {
+      const int fb = it >> 3, p = it & 7;
+      int s, w, c;
+      TI::face_of(g, list[f0 + fb] >> 1, s, w, c);
+      const int64_t h = TI::hidx(g, s - 3 + p, w, c);
+      const double dd = B.d[h], uu = B.u[h], vv = B.v[h], ww = B.w[h];
+      const double cc = sqrt(B.T[h]) / rma;
+      double A1, A2, A3;
+      if (FAST || B.M.uniform) {
+        A1 = B.M.u[q1]; A2 = B.M.u[q1 + 1]; A3 = B.M.u[q1 + 2];
+      } else {
+        A1 = B.M.p[q1][h]; A2 = B.M.p[q1 + 1][h]; A3 = B.M.p[q1 + 2][h];
+      }
+      double fp[5], fm[5];
+      if (S.flux_split == 1) split_sw(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+      else split_llf(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        sF[fb][p][m] = fp[m];
+        sF[fb][p][5 + m] = fm[m];
+      }
+      if (CHAR && (p == 3 || p == 4)) {
+        double *q = sQ[fb][p - 3];
+        q[0] = uu; q[1] = vv; q[2] = ww; q[3] = cc;
+        q[4] = A1; q[5] = A2; q[6] = A3;
+      }
+    }
+    __syncthreads();
+    if (CHAR) {
+      // basis from the face-averaged state (OCFD_inviscous_character.f90:200-236)
+      for (int fb = tid; fb < nf; fb += NT) {
+        const double *qa = sQ[fb][0], *qb = sQ[fb][1];
+        const double u1 = (qa[0] + qb[0]) * 0.5, v1 = (qa[1] + qb[1]) * 0.5;
+        const double w1 = (qa[2] + qb[2]) * 0.5, c1 = (qa[3] + qb[3]) * 0.5;
+        const double a1 = (qa[4] + qb[4]) * 0.5, a2 = (qa[5] + qb[5]) * 0.5, a3 = (qa[6] + qb[6]) * 0.5;
+        CharBasis Bs;
+        char_basis(gamma, u1, v1, w1, c1, a1, a2, a3, Bs);
+        double *o = sB[fb];
+        o[0] = Bs.u1; o[1] = Bs.v1; o[2] = Bs.w1; o[3] = Bs.c1; o[4] = Bs.n1; o[5] = Bs.n2; o[6] = Bs.n3;
+        o[7] = Bs.l1; o[8] = Bs.l2; o[9] = Bs.l3; o[10] = Bs.m1; o[11] = Bs.m2; o[12] = Bs.m3;
+        o[13] = Bs.un; o[14] = Bs.ul; o[15] = Bs.um; o[16] = Bs.V2; o[17] = Bs.KK; o[18] = Bs.KK1;
+        o[19] = Bs.X1; o[20] = Bs.H;
+      }
+      __syncthreads();
+      auto basis = [&](int fb) {
+        const double *o = sB[fb];
+        CharBasis Bs;
+        Bs.u1 = o[0]; Bs.v1 = o[1]; Bs.w1 = o[2]; Bs.c1 = o[3]; Bs.n1 = o[4]; Bs.n2 = o[5]; Bs.n3 = o[6];
+        Bs.l1 = o[7]; Bs.l2 = o[8]; Bs.l3 = o[9]; Bs.m1 = o[10]; Bs.m2 = o[11]; Bs.m3 = o[12];
+        Bs.un = o[13]; Bs.ul = o[14]; Bs.um = o[15]; Bs.V2 = o[16]; Bs.KK = o[17]; Bs.KK1 = o[18];
+        Bs.X1 = o[19]; Bs.H = o[20];
+        return Bs;
+      };
+      // one characteristic row per lane: project, reconstruct (flux_x1/x2)
+      for (int it = tid; it < nf * 5; it += NT) {
+        const int fb = it / 5, r = it - 5 * fb;
+        const int code = list[f0 + fb];
+        const int scm = (code & 1) ? 3 : 2;
+        int s, w, c;
+        TI::face_of(g, code >> 1, s, w, c);
+        const CharBasis Bs = basis(fb);
+        double sr5[5];
+        char_S_row(Bs, r, sr5);
+        // stencils of the hybrid's WENO7 / WENO5 faces (Ka1_H2.. / Ka1_H3..)
+        const int a1h = scm == 2 ? -3 : -2, b1h = scm == 2 ? 3 : 2;
+        const int a2h = scm == 2 ? -2 : -1, b2h = scm == 2 ? 4 : 3;
+        double xp[8], xm[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          xp[k] = sr5[0] * sF[fb][k][0];
+          xm[k] = sr5[0] * sF[fb][k][5];
+        }
+#pragma unroll
+        for (int cmp = 1; cmp < 5; cmp++) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            xp[k] = xp[k] + sr5[cmp] * sF[fb][k][cmp];
+            xm[k] = xm[k] + sr5[cmp] * sF[fb][k][5 + cmp];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (k < a1h + 3 || k > b1h + 3) xp[k] = 0.0;
+          if (k < a2h + 3 || k > b2h + 3) xm[k] = 0.0;
+        }
+        const double h1 = FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_char<SCH_HYBRID>(S, xp, s, n, AXIS, scm);
+        const double h2 = FAST ? inner_m<SCH_HYBRID>(S, xm, scm) : face_m_char<SCH_HYBRID>(S, xm, s, n, AXIS, scm);
+        sG[fb][r] = h1 + h2;
+      }
+      __syncthreads();
+      // back to conservative components (S1 * h0)
+      for (int fb = tid; fb < nf; fb += NT) {
+        const CharBasis Bs = basis(fb);
+        double hh[5];
+        char_back(Bs, sG[fb], hh);
+        double *o = out + (int64_t)(f0 + fb) * NH;
+#pragma unroll
+        for (int m = 0; m < 5; m++) o[m] = hh[m];
+      }
+    } else {
+      // one component per lane: flux+ and flux- of the face
+      for (int it = tid; it < nf * 5; it += NT) {
+        const int fb = it / 5, m = it - 5 * fb;
+        const int code = list[f0 + fb];
+        const int scm = (code & 1) ? 3 : 2;
+        int s, w, c;
+        TI::face_of(g, code >> 1, s, w, c);
+        double xp[8], xm[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          xp[k] = sF[fb][k][m];
+          xm[k] = sF[fb][k][5 + m];
+        }
+        double *o = out + (int64_t)(f0 + fb) * NH;
+        o[m] = FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_nonchar<SCH_HYBRID>(S, xp, s, n, AXIS, scm);
+        o[5 + m] = FAST ? inner_m<SCH_HYBRID>(S, xm, scm) : face_m_nonchar<SCH_HYBRID>(S, xm, s, n, AXIS, scm);
+      }
+    }
+    __syncthreads();  // the next batch overwrites the LDS arrays
+  }
+}
+
+template <int AXIS>
+hipError_t weno_axis(const Block &B, hipStream_t st) {
+  // enough workgroups to fill the chip; each strides over the batches
+  const dim3 grid(1024);
+  const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
+  if (B.S.is_char) {
+    if (fast) hipLaunchKernelGGL((k_weno_faces<AXIS, true, true>), grid, dim3(NT), 0, st, B);
+    else hipLaunchKernelGGL((k_weno_faces<AXIS, true, false>), grid, dim3(NT), 0, st, B);
+  } else {
+    if (fast) hipLaunchKernelGGL((k_weno_faces<AXIS, false, true>), grid, dim3(NT), 0, st, B);
+    else hipLaunchKernelGGL((k_weno_faces<AXIS, false, false>), grid, dim3(NT), 0, st, B);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_hybrid_lists(const Block &B, hipStream_t st) {
+  const Geom &g = B.g;
+  hipLaunchKernelGGL(k_hyb_list<0>, dim3((unsigned)((TileIndex<0>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
+  hipLaunchKernelGGL(k_hyb_list<1>, dim3((unsigned)((TileIndex<1>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
+  hipLaunchKernelGGL(k_hyb_list<2>, dim3((unsigned)((TileIndex<2>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st) {
+  if (axis == 0) return weno_axis<0>(B, st);
+  if (axis == 1) return weno_axis<1>(B, st);
+  return weno_axis<2>(B, st);
+}
+
+}  // namespace ocfd

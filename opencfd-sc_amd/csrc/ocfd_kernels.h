@@ -15,6 +15,54 @@ struct Block {
   double *f, *fn, *du;        // 5 planes each
   double *rhyb;               // halo'd sensor (hybrid scheme only)
   double *ev;                 // 12 interior planes: Ev1(1:4), Ev2(1:4), Ev3(1:4)
+  double *trash;              // 16 x 256 doubles: stores of lanes outside the block
+  // hybrid scheme, per sweep axis: the faces that use WENO7/WENO5 (face id
+  // * 2 + (WENO5)), their count, the face -> list slot map (-1: linear) and
+  // the fluxes of the listed faces (5 per face characteristic, 10 otherwise)
+  int *hlist[3], *hmap[3], *hcount;
+  double *wflux[3];
+  int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
+};
+
+// Index arithmetic of a sweep along AXIS: point s of line w, third index c
+// (x: lines j, third k; y: lines i, third k; z: lines i, third j).
+template <int AXIS>
+struct TileIndex {
+  // halo'd index of point s on line w, third index c
+  __device__ static inline int64_t hidx(const Geom &g, int s, int w, int c) {
+    if (AXIS == 0) return g.hi(s, w, c);
+    if (AXIS == 1) return g.hi(w, s, c);
+    return g.hi(w, c, s);
+  }
+  __device__ static inline int64_t iidx(const Geom &g, int s, int w, int c) {
+    if (AXIS == 0) return g.ii(s, w, c);
+    if (AXIS == 1) return g.ii(w, s, c);
+    return g.ii(w, c, s);
+  }
+  __host__ __device__ static inline int nwidth(const Geom &g) { return AXIS == 0 ? g.ny : g.nx; }
+  __host__ __device__ static inline int nthird(const Geom &g) { return AXIS == 2 ? g.ny : g.nz; }
+  // faces s = 0..n of every line, in the order the sweep tiles read them
+  // (x: s fastest; y, z: the line index fastest)
+  __host__ __device__ static inline int64_t nfaces(const Geom &g) {
+    return (int64_t)(g.n(AXIS) + 1) * nwidth(g) * nthird(g);
+  }
+  __device__ static inline int64_t face_id(const Geom &g, int s, int w, int c) {
+    if (AXIS == 0) return ((int64_t)(c - 1) * g.ny + (w - 1)) * (g.nx + 1) + s;
+    return ((int64_t)(c - 1) * (g.n(AXIS) + 1) + s) * g.nx + (w - 1);
+  }
+  __device__ static inline void face_of(const Geom &g, int64_t fid, int &s, int &w, int &c) {
+    if (AXIS == 0) {
+      s = (int)(fid % (g.nx + 1));
+      const int64_t r = fid / (g.nx + 1);
+      w = (int)(r % g.ny) + 1;
+      c = (int)(r / g.ny) + 1;
+    } else {
+      w = (int)(fid % g.nx) + 1;
+      const int64_t r = fid / g.nx;
+      s = (int)(r % (g.n(AXIS) + 1));
+      c = (int)(r / (g.n(AXIS) + 1)) + 1;
+    }
+  }
 };
 
 enum InvMode : int { INV_ASSIGN = 0, INV_ACCUM = 1, INV_FINAL = 2 };
@@ -26,6 +74,8 @@ struct SweepTile {
 
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, hipStream_t st);
+hipError_t launch_hybrid_lists(const Block &B, hipStream_t st);
+hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st);
 hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st);
 hipError_t launch_sensor(const Block &B, hipStream_t st);
 hipError_t launch_sensor2(const Block &B, double *work, hipStream_t st);

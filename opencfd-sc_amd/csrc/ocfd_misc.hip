@@ -103,7 +103,7 @@ __global__ __launch_bounds__(NT) void k_sensor2a(const Block B, const double p00
       const int64_t st = g.hstride(a);
       const int s = a == 0 ? i : (a == 1 ? j : k);
       auto at = [&](int q) { const int64_t x = h + (int64_t)q * st; return p00 * B.d[x] * B.T[x]; };
-      dp[a] = dxyz0_point(at, s, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+      dp[a] = dxyz0_point(at, s, g.n(a), B.P.dc[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
     }
     const Metrics &M = B.M;
     const double px = dp[0] * M.get(M_AKX, h) + dp[1] * M.get(M_AIX, h) + dp[2] * M.get(M_ASX, h);
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(NT) void k_ana_box(const Block B, double *partial, 
           const double *arr = fld[c];
           auto at = [&](int q) { return arr[h + (int64_t)q * st]; };
           gr[c * 3 + a] =
-              dxyz0_point(at, sidx, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+              dxyz0_point(at, sidx, g.n(a), B.P.dc[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
         }
       const Metrics &M = B.M;
       auto X = [&](int c) { return gr[c * 3] * M.get(M_AKX, h) + gr[c * 3 + 1] * M.get(M_AIX, h) + gr[c * 3 + 2] * M.get(M_ASX, h); };
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(NT) void k_jacobian(const Block B, const double *X,
       const int s = a == 0 ? i : (a == 1 ? j : k);
       const double *arr = C[c];
       auto at = [&](int q) { return arr[h + (int64_t)q * st]; };
-      dd[c * 3 + a] = dxyz0_point(at, s, g.n(a), B.P.h[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
+      dd[c * 3 + a] = dxyz0_point(at, s, g.n(a), B.P.dc[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
     }
   jac_point(mp.p, h, dd[0], dd[1], dd[2], dd[3], dd[4], dd[5], dd[6], dd[7], dd[8]);
 }

@@ -1,0 +1,412 @@
+// ocfd_sweep.h -- inviscid right-hand side: one pencil sweep per axis.
+//
+// Replaces du_inviscous (OCFD_inviscous.f90:53-203) and du_inviscous_Character
+// (OCFD_inviscous_character.f90:3-159).  One launch per axis; a workgroup owns
+// a tile of TL cells along the sweep axis on TW neighbouring lines:
+//   phase 1  every point of the tile plus its 4-point halo on both sides:
+//            load d,u,v,w,T (+ normalised metrics), sound speed, flux-vector
+//            splitting -> fp(5), fm(5) in LDS (coalesced global loads);
+//   phase 2  every face of the tile: reconstruction (WENO5/WENO7/OMP6/UD7L/
+//            hybrid, characteristic projection when requested, block-edge
+//            closures) -> face fluxes in LDS;
+//   phase 3  every cell: flux difference / h written to du (x: assign,
+//            y: accumulate, z: accumulate, times -Ajac, plus mass force).
+// Axis x tiles run along i (contiguous); y and z tiles put TW consecutive i on
+// the lines so every global access is a contiguous run of i.
+// R is the arithmetic of the splitting and of the faces the sweep computes
+// (double; float for the mixed-precision form); primitives, du and the
+// hybrid scheme's precomputed WENO faces stay double.  Each translation unit
+// that includes this header instantiates one R.
+#pragma once
+#include "ocfd_kernels.h"
+
+namespace ocfd {
+
+namespace {
+
+constexpr int NTHREADS = 256;
+
+// LDS element (component m, point or face p, line wl) of a tile array with NPT
+// entries per line: x tiles keep the points of one line contiguous, y and z
+// tiles the lines of one point, so that consecutive lanes touch consecutive
+// doubles in every phase.
+template <int AXIS, int NPT, int TW>
+__device__ constexpr inline int lidx(int m, int p, int wl) {
+  return AXIS == 0 ? (m * TW + wl) * NPT + p : (m * NPT + p) * TW + wl;
+}
+
+// FAST: the sweep axis is periodic (no block-edge closures) and the metrics
+// are uniform, both known at compile time.
+template <int AXIS, int SCH, bool CHAR, int TL, int TW, bool FAST, typename R>
+__global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int mode) {
+  typedef TileIndex<AXIS> TI;
+  constexpr int NP = TL + 8;  // points s0-4 .. s0+TL+3
+  constexpr int NF = TL + 1;  // faces  s0-1 .. s0+TL-1
+  constexpr bool HYB = (SCH == SCH_HYBRID);
+  constexpr int NH = CHAR ? 5 : 10;
+  constexpr int NI1 = (NP * TW + NTHREADS - 1) / NTHREADS;      // points per thread
+  constexpr int NI3 = (5 * TL * TW + NTHREADS - 1) / NTHREADS;  // du entries per thread
+  // characteristic inputs per point: u, v, w, c (+ the three normalised
+  // metrics when they vary in space)
+  const bool unif = FAST || B.M.uniform != 0;
+  const int nq = unif ? 4 : 7;
+  extern __shared__ double smem[];
+
+  const Geom &g = B.g;
+  const int n = g.n(AXIS);
+  const int nw = TI::nwidth(g), nc = TI::nthird(g);
+  const int nts = (n + TL - 1) / TL, ntw = (nw + TW - 1) / TW;
+  int b = blockIdx.x;
+  const int ts = b % nts; b /= nts;
+  const int tw = b % ntw;
+  const int c = b / ntw + 1;
+  const int s0 = 1 + ts * TL, w0 = 1 + tw * TW;
+  const int tid = threadIdx.x;
+  const double gamma = B.P.gamma, rma = B.P.ma;
+  const int q1 = M_AKX1 + 3 * AXIS;
+  const SchemeParams &S = B.S;
+  const int lo = S.bidx[AXIS][0], hi = S.bidx[AXIS][1];
+  const int sl = S.sb[2 * AXIS], sr = S.sb[2 * AXIS + 1];
+
+  // point handled by phase-1 slot r of this thread
+  auto p1 = [&](int r, int &p, int &wl) {
+    const int it = tid + r * NTHREADS;
+    if (AXIS == 0) { p = it % NP; wl = it / NP; } else { wl = it % TW; p = it / TW; }
+    return it < NP * TW && s0 - 4 + p <= n + LAP && w0 + wl <= nw;
+  };
+  // du entry handled by phase-3 slot r
+  auto p3 = [&](int r, int &sl_, int &wl, int &m) {
+    const int it = tid + r * NTHREADS;
+    if (AXIS == 0) { sl_ = it % TL; wl = (it / TL) % TW; m = it / (TL * TW); }
+    else { wl = it % TW; sl_ = (it / TW) % TL; m = it / (TL * TW); }
+    return it < 5 * TL * TW && s0 + sl_ <= n && w0 + wl <= nw;
+  };
+
+  // primitives (+ sensor) of the tile's points
+  double pq[NI1][5];
+  {
+#pragma unroll
+    for (int r = 0; r < NI1; r++) {
+      int p, wl;
+      if (!p1(r, p, wl)) continue;
+      const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
+      pq[r][0] = B.d[h];
+      pq[r][1] = B.u[h];
+      pq[r][2] = B.v[h];
+      pq[r][3] = B.w[h];
+      pq[r][4] = B.T[h];
+    }
+  }
+  // du entries of the tile (accumulating modes), fetched before the face
+  // fluxes are computed
+  double pdu[NI3];
+
+  R *sF = reinterpret_cast<R *>(smem);             // fp(5), fm(5)      [10][NP][TW]
+  // the characteristic inputs are needed only where the sweep itself
+  // projects, i.e. not for the hybrid scheme (its WENO faces come precomputed)
+  constexpr bool PROJ = CHAR && !HYB;
+  R *sQ = sF + 10 * NP * TW;                       // char inputs       [nq][NP][TW]
+  R *sH = sQ + (PROJ ? nq * NP * TW : 0);          // face fluxes       [NH][NF][TW]
+  int *sS = reinterpret_cast<int *>(sH + NH * NF * TW);  // hybrid: list slot of each face [NF][TW]
+  // ---- phase 1: split fluxes at the tile's points ----
+#pragma unroll
+  for (int r = 0; r < NI1; r++) {
+    int p, wl;
+    if (!p1(r, p, wl)) continue;
+    const R dd = R(pq[r][0]), uu = R(pq[r][1]), vv = R(pq[r][2]), ww = R(pq[r][3]);
+    const R cc = sqrt(R(pq[r][4])) / R(rma);  // OCFD_inviscous.f90:67
+    R A1, A2, A3;
+    if (unif) {
+      A1 = R(B.M.u[q1]); A2 = R(B.M.u[q1 + 1]); A3 = R(B.M.u[q1 + 2]);
+    } else {
+      const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
+      A1 = R(B.M.p[q1][h]); A2 = R(B.M.p[q1 + 1][h]); A3 = R(B.M.p[q1 + 2][h]);
+    }
+    R fp[5], fm[5];
+    if (S.flux_split == 1) split_sw<R>(R(gamma), cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+    else split_llf<R>(R(gamma), cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      sF[lidx<AXIS, NP, TW>(m, p, wl)] = fp[m];
+      sF[lidx<AXIS, NP, TW>(5 + m, p, wl)] = fm[m];
+    }
+    if (PROJ) {
+      sQ[lidx<AXIS, NP, TW>(0, p, wl)] = uu;
+      sQ[lidx<AXIS, NP, TW>(1, p, wl)] = vv;
+      sQ[lidx<AXIS, NP, TW>(2, p, wl)] = ww;
+      sQ[lidx<AXIS, NP, TW>(3, p, wl)] = cc;
+      if (!unif) {
+        sQ[lidx<AXIS, NP, TW>(4, p, wl)] = A1;
+        sQ[lidx<AXIS, NP, TW>(5, p, wl)] = A2;
+        sQ[lidx<AXIS, NP, TW>(6, p, wl)] = A3;
+      }
+    }
+  }
+  if (HYB) {
+    // WENO faces of the hybrid scheme come precomputed (k_weno_faces)
+    for (int it = tid; it < NF * TW; it += NTHREADS) {
+      int f, wl;
+      if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
+      const int s = s0 - 1 + f, w = w0 + wl;
+      int slot = -1;
+      if (s <= n && w <= nw) slot = B.hmap[AXIS][TI::face_id(g, s, w, c)];
+      sS[lidx<AXIS, NF, TW>(0, f, wl)] = slot;
+      if (slot >= 0) {
+        const double *wf = B.wflux[AXIS] + (int64_t)slot * NH;
+#pragma unroll
+        for (int m = 0; m < NH; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = R(wf[m]);
+      }
+    }
+  }
+  __syncthreads();
+
+  if (mode != INV_ASSIGN) {
+#pragma unroll
+    for (int r = 0; r < NI3; r++) {
+      int sl_, wl, m;
+      if (!p3(r, sl_, wl, m)) continue;
+      pdu[r] = B.du[TI::iidx(g, s0 + sl_, w0 + wl, c) + (int64_t)m * g.ni];
+    }
+  }
+
+  // ---- phase 2: face fluxes ----
+  // flux+/- of component m at face f (non-characteristic)
+  auto face_nc = [&](int m, int f, int wl, int s, int scm) {
+    R xp[8], xm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      xp[k] = sF[lidx<AXIS, NP, TW>(m, f + k, wl)];
+      xm[k] = sF[lidx<AXIS, NP, TW>(5 + m, f + k, wl)];
+    }
+    sH[lidx<AXIS, NF, TW>(m, f, wl)] = FAST ? inner_p<SCH>(S, xp, scm) : face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
+    sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] = FAST ? inner_m<SCH>(S, xm, scm) : face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
+  };
+#define FPV(m, k) sF[lidx<AXIS, NP, TW>((m), f + (k), wl)]
+#define FMV(m, k) sF[lidx<AXIS, NP, TW>(5 + (m), f + (k), wl)]
+  // all five components of face f, projected on the characteristic basis of
+  // the face unless the face is linear (flux_charteric,
+  // OCFD_inviscous_character.f90:190-290)
+  // characteristic basis of face f from the face-averaged state
+  // (OCFD_inviscous_character.f90:200-236)
+  auto basis_of = [&](int f, int wl, CharBasisT<R> &Bs) {
+#define QV(q, k) sQ[lidx<AXIS, NP, TW>((q), f + (k), wl)]
+    const R u1 = (QV(0, 3) + QV(0, 4)) * R(0.5), v1 = (QV(1, 3) + QV(1, 4)) * R(0.5);
+    const R w1 = (QV(2, 3) + QV(2, 4)) * R(0.5), c1 = (QV(3, 3) + QV(3, 4)) * R(0.5);
+    R a1, a2, a3;
+    if (unif) {
+      // average of two equal values, as the reference computes it
+      a1 = (R(B.M.u[q1]) + R(B.M.u[q1])) * R(0.5);
+      a2 = (R(B.M.u[q1 + 1]) + R(B.M.u[q1 + 1])) * R(0.5);
+      a3 = (R(B.M.u[q1 + 2]) + R(B.M.u[q1 + 2])) * R(0.5);
+    } else {
+      a1 = (QV(4, 3) + QV(4, 4)) * R(0.5);
+      a2 = (QV(5, 3) + QV(5, 4)) * R(0.5);
+      a3 = (QV(6, 3) + QV(6, 4)) * R(0.5);
+    }
+#undef QV
+    char_basis<R>(R(gamma), u1, v1, w1, c1, a1, a2, a3, Bs);
+  };
+  // characteristic flux h0(r) = h+(r) + h-(r) of face f: row r of S applied
+  // to the split fluxes on the scheme's stencil, then reconstructed
+  // (OCFD_inviscous_character.f90:250-282)
+  auto char_row = [&](const CharBasisT<R> &Bs, int f, int wl, int s, int scm, int r) {
+    int a1h = S.ka1, b1h = S.kb1, a2h = S.ka2, b2h = S.kb2;
+    if (HYB) {
+      if (scm == 2) { a1h = -3; b1h = 3; a2h = -2; b2h = 4; }
+      else if (scm == 3) { a1h = -2; b1h = 2; a2h = -1; b2h = 3; }
+    }
+    R xp[8], xm[8], sr5[5];
+    char_S_row(Bs, r, sr5);
+    // xp(k) = sum over components c of S(r,c) * fp(c,k), summed in the
+    // reference's order c = 1..5; outside the scheme's stencil the value is 0
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      xp[k] = sr5[0] * FPV(0, k);
+      xm[k] = sr5[0] * FMV(0, k);
+    }
+#pragma unroll
+    for (int c = 1; c < 5; c++) {
+      // keeps the scheduler from fetching all 80 stencil values at once
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        xp[k] = xp[k] + sr5[c] * FPV(c, k);
+        xm[k] = xm[k] + sr5[c] * FMV(c, k);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (k < a1h + 3 || k > b1h + 3) xp[k] = R(0.0);
+      if (k < a2h + 3 || k > b2h + 3) xm[k] = R(0.0);
+    }
+    const R h1 = FAST ? inner_p<SCH>(S, xp, scm) : face_p_char<SCH>(S, xp, s, n, AXIS, scm);
+    const R h2 = FAST ? inner_m<SCH>(S, xm, scm) : face_m_char<SCH>(S, xm, s, n, AXIS, scm);
+    return h1 + h2;
+  };
+  // component m of a linear (hybrid UD7L) face: no projection
+  auto linear_comp = [&](int f, int wl, int s, int m) {
+    R xp[8], xm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { xp[k] = FPV(m, k); xm[k] = FMV(m, k); }
+    const R h1 = FAST ? inner_p<SCH>(S, xp, 1) : face_p_char<SCH>(S, xp, s, n, AXIS, 1);
+    const R h2 = FAST ? inner_m<SCH>(S, xm, 1) : face_m_char<SCH>(S, xm, s, n, AXIS, 1);
+    return h1 + h2;
+  };
+  // non-reflecting closure of component m (OCFD_inviscous_character.f90:295-309);
+  // point t is k = t - s + 3
+  auto closure_comp = [&](int f, int wl, int s, int m) {
+    if (s == 0)
+      return (R(2.0) * FPV(m, 4) + R(5.0) * FPV(m, 5) - FPV(m, 6) + R(2.0) * FMV(m, 6) - R(7.0) * FMV(m, 5) +
+              R(11.0) * FMV(m, 4)) / R(6.0);
+    return (R(2.0) * FPV(m, 1) - R(7.0) * FPV(m, 2) + R(11.0) * FPV(m, 3) + R(2.0) * FMV(m, 3) +
+            R(5.0) * FMV(m, 2) - FMV(m, 1)) / R(6.0);
+  };
+  // all five components of face f, projected on the characteristic basis of
+  // the face (flux_charteric, OCFD_inviscous_character.f90:190-290)
+  auto face_ch = [&](int f, int wl, int s, int scm) {
+    CharBasisT<R> Bs;
+    basis_of(f, wl, Bs);
+    R h0[5], hh[5];
+    for (int r = 0; r < 5; r++) h0[r] = char_row(Bs, f, wl, s, scm, r);
+    char_back(Bs, h0, hh);
+#pragma unroll
+    for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = hh[m];
+  };
+  auto listed = [&](int f, int wl) { return sS[lidx<AXIS, NF, TW>(0, f, wl)] >= 0; };
+  // face work item -> (component, face, line)
+  auto item_nc = [&](int it, int &m, int &f, int &wl) {
+    if (AXIS == 0) { f = it % NF; wl = (it / NF) % TW; } else { wl = it % TW; f = (it / TW) % NF; }
+    m = it / (TW * NF);
+  };
+  auto item_ch = [&](int it, int &f, int &wl) {
+    if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
+  };
+  const bool closure_lo = !FAST && lo == 1 && sl == 0, closure_hi = !FAST && hi == 1 && sr == 0;
+  if (!CHAR) {
+    for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
+      int m, f, wl;
+      item_nc(it, m, f, wl);
+      const int s = s0 - 1 + f;
+      if (s > n || w0 + wl > nw) continue;
+      if (HYB && listed(f, wl)) continue;
+      face_nc(m, f, wl, s, HYB ? 1 : 0);
+    }
+  } else if (!HYB) {
+    for (int it = tid; it < NF * TW; it += NTHREADS) {
+      int f, wl;
+      item_ch(it, f, wl);
+      const int s = s0 - 1 + f;
+      if (s > n || w0 + wl > nw) continue;
+      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+        for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
+        continue;
+      }
+      face_ch(f, wl, s, 0);
+    }
+  } else {
+    // hybrid, characteristic: the linear and closure faces, per component
+    for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
+      int m, f, wl;
+      item_nc(it, m, f, wl);
+      const int s = s0 - 1 + f;
+      if (s > n || w0 + wl > nw) continue;
+      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+        sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
+        continue;
+      }
+      if (!listed(f, wl)) sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+    }
+  }
+#undef FPV
+#undef FMV
+  __syncthreads();
+
+  // ---- phase 3: flux difference into du ----
+  const double hinv = B.P.hinv[AXIS];
+#pragma unroll
+  for (int r = 0; r < NI3; r++) {
+    int sl_, wl, m;
+    if (!p3(r, sl_, wl, m)) continue;
+    const int s = s0 + sl_, w = w0 + wl;
+    const int f = sl_ + 1;
+    R val;
+    if (!CHAR) {
+      val = (sH[lidx<AXIS, NF, TW>(m, f, wl)] - sH[lidx<AXIS, NF, TW>(m, f - 1, wl)] +
+             sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] - sH[lidx<AXIS, NF, TW>(5 + m, f - 1, wl)]) * R(hinv);
+    } else {
+      val = (sH[lidx<AXIS, NF, TW>(m, f, wl)] - sH[lidx<AXIS, NF, TW>(m, f - 1, wl)]) * R(hinv);
+    }
+    const int64_t q = TI::iidx(g, s, w, c) + (int64_t)m * g.ni;
+    if (mode == INV_ASSIGN) {
+      B.du[q] = double(val);
+    } else if (mode == INV_ACCUM) {
+      B.du[q] = pdu[r] + double(val);
+    } else {
+      const int64_t h = TI::hidx(g, s, w, c);
+      double rr = -(pdu[r] + double(val)) * (FAST ? B.M.u[M_AJAC] : B.M.get(M_AJAC, h));
+      if (B.P.if_mass_force == 1 && m > 0) {
+        // OCFD_inviscous.f90:189-201
+        const double *mf = B.P.mass_force;
+        if (m < 4) rr = rr + B.d[h] * mf[m - 1];
+        else rr = rr + B.d[h] * (B.u[h] * mf[0] + B.v[h] * mf[1] + B.w[h] * mf[2]);
+      }
+      B.du[q] = rr;
+    }
+  }
+}
+
+template <int TL, int TW, bool CHAR, bool HYB, typename R>
+constexpr size_t sweep_lds(int nq) {
+  return sizeof(R) * (10 * (TL + 8) * TW + (CHAR && !HYB ? nq * (TL + 8) * TW : 0) + (CHAR ? 5 : 10) * (TL + 1) * TW) +
+         (HYB ? sizeof(int) * (TL + 1) * TW : 0);
+}
+
+template <int AXIS, int SCH, bool CHAR, int TL, int TW, bool FAST, typename R>
+hipError_t launch_one(const Block &B, int mode, hipStream_t st) {
+  constexpr bool HYB = (SCH == SCH_HYBRID);
+  static_assert(sweep_lds<TL, TW, CHAR, HYB, R>(7) <= 64 * 1024, "tile too large for LDS");
+  const size_t lds = sweep_lds<TL, TW, CHAR, HYB, R>(B.M.uniform ? 4 : 7);
+  const Geom &g = B.g;
+  const int n = g.n(AXIS);
+  const int nw = AXIS == 0 ? g.ny : g.nx;
+  const int nc = AXIS == 2 ? g.ny : g.nz;
+  const int64_t blocks = (int64_t)((n + TL - 1) / TL) * ((nw + TW - 1) / TW) * nc;
+  hipLaunchKernelGGL((k_inviscid<AXIS, SCH, CHAR, TL, TW, FAST, R>), dim3((unsigned)blocks), dim3(NTHREADS), lds, st, B,
+                     mode);
+  return hipGetLastError();
+}
+
+template <int AXIS, bool CHAR, bool FAST, typename R>
+hipError_t dispatch_scheme(const Block &B, int mode, hipStream_t st) {
+  constexpr int TL = AXIS == 0 ? 64 : 32;
+  constexpr int TW = AXIS == 0 ? 4 : 8;
+  switch (B.S.scheme_invis) {
+    case SCH_WENO5: return launch_one<AXIS, SCH_WENO5, CHAR, TL, TW, FAST, R>(B, mode, st);
+    case SCH_WENO7: return launch_one<AXIS, SCH_WENO7, CHAR, TL, TW, FAST, R>(B, mode, st);
+    case SCH_OMP6: return launch_one<AXIS, SCH_OMP6, CHAR, TL, TW, FAST, R>(B, mode, st);
+    case SCH_UD7L: return launch_one<AXIS, SCH_UD7L, CHAR, TL, TW, FAST, R>(B, mode, st);
+    case SCH_HYBRID: return launch_one<AXIS, SCH_HYBRID, CHAR, TL, TW, FAST, R>(B, mode, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int AXIS, typename R>
+hipError_t dispatch_axis(const Block &B, int mode, hipStream_t st) {
+  const bool ch = B.S.is_char != 0;
+  const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
+  if (fast)
+    return ch ? dispatch_scheme<AXIS, true, true, R>(B, mode, st) : dispatch_scheme<AXIS, false, true, R>(B, mode, st);
+  return ch ? dispatch_scheme<AXIS, true, false, R>(B, mode, st) : dispatch_scheme<AXIS, false, false, R>(B, mode, st);
+}
+
+// the three axes of one real type
+template <typename R>
+hipError_t launch_sweep(const Block &B, int axis, int mode, hipStream_t st) {
+  if (axis == 0) return dispatch_axis<0, R>(B, mode, st);
+  if (axis == 1) return dispatch_axis<1, R>(B, mode, st);
+  return dispatch_axis<2, R>(B, mode, st);
+}
+
+}  // namespace
+
+}  // namespace ocfd

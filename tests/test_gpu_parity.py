@@ -145,6 +145,65 @@ def test_uniform_metrics_mode_matches_oracle():
         assert_same(b.download_f(), ob.download_f(), "uniform-metric f after 3 steps")
 
 
+# periodic box + uniform metrics selects the specialised kernels (no block-edge
+# closures, metrics as constants); non-cubic 40x28x20 blocks leave partial
+# tiles on every axis
+UNIFORM = [
+    dict(seed=51, scheme="HYBRID", split=2, char=1, jump=0.5, hybrid=(0.002, 0.02)),
+    dict(seed=52, scheme="WENO7", split=2, char=1),
+    dict(seed=53, scheme="OMP6", split=1, char=1, noise=0.03),
+    dict(seed=54, scheme="UD7L", split=2, char=0, vis="CD8"),
+    dict(seed=55, scheme="HYBRID", split=1, char=0, jump=0.5, hybrid=(0.002, 0.02)),
+    dict(seed=56, scheme="WENO5", split=1, char=0, filters=[[1, 1, 1, 1, 1, 40, 1, 28, 1, 20, 1, 0.2, 0.0]]),
+]
+
+
+@pytest.mark.parametrize("kw", UNIFORM, ids=[m["scheme"] + str(m["seed"]) for m in UNIFORM])
+def test_uniform_metrics_periodic_box_against_oracle(kw):
+    case = cases.random_case(shape=(20, 28, 40), **kw)
+    case.uniform_metrics = cases.uniform_box_metrics(case.cfg.nx, case.cfg.ny, case.cfg.nz)
+    case.cfg.metric_mode = 1
+    ob = pyoracle.setup_oracle(case, uniform=True)
+    with gpu_block(case, uniform=True) as b:
+        for blk in (ob, b):
+            blk.save_fn()
+            blk.exchange_primitives()
+            if case.cfg.scheme_invis == "HYBRID":
+                blk.comput_rhybrid()
+            (blk.du_inviscous_character if case.cfg.if_scheme_character else blk.du_inviscous)()
+            blk.du_viscous()
+        assert_same(b.download_du(), ob.download_du(), "du (uniform metrics)")
+        ob.step(3)
+        b.step(3)
+        assert_same(b.download_f(), ob.download_f(), "f after 3 steps (uniform metrics)")
+        pg, po = b.download_primitives(), ob.download_primitives()
+        for q in range(5):
+            assert_same(interior(pg[q]), interior(po[q]), "primitive %d" % q)
+
+
+def test_graph_replay_equals_eager_steps(monkeypatch):
+    """ocfd_step replays a captured hipGraph of one time step; the replays,
+    with eager work and synchronisations between them, equal eager launches."""
+    case = cases.random_case(shape=(16, 20, 24), seed=61, scheme="HYBRID", split=2, char=1, jump=0.5,
+                             hybrid=(0.002, 0.02))
+    monkeypatch.setenv("OCFD_NO_GRAPH", "1")
+    eager = gpu_block(case)
+    monkeypatch.delenv("OCFD_NO_GRAPH")
+    with eager, gpu_block(case) as g:
+        for rep in range(4):
+            eager.step(1)
+            g.step(1)
+            g.show_flow_msg()
+            g.synchronize()
+            assert_same(g.download_f(), eager.download_f(), "f after replay %d" % rep)
+        eager.step(3)
+        g.step(3)
+        assert_same(g.download_f(), eager.download_f(), "f after 3 more replays")
+        pg, pe = g.download_primitives(), eager.download_primitives()
+        for q in range(5):
+            assert_same(pg[q], pe[q], "halo'd primitive %d" % q)
+
+
 def test_negative_temperature_trap_and_repair():
     """Three isolated cells get negative energy: comput_duvwT reports them and
     replaces T by the mean of the six neighbours (OCFD_handle_NegativeT.f90)."""
