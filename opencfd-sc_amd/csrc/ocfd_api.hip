@@ -123,7 +123,8 @@ int set_params(ocfd_handle *h) {
   if (c.flux_splitting != 1 && c.flux_splitting != 2) return fail("Flux_Splitting must be 1 or 2");
   if (c.ibc != 0 && c.ibc != 1 && c.ibc != 99) return fail("BoundaryCondition not supported");
   if (c.nfiltering < 0 || c.nfiltering > 10) return fail("NFiltering must be <= 10");
-  if (c.precision != OCFD_PREC_FP64) return fail("only OCFD_PREC_FP64 is available in this build");
+  if (c.precision != OCFD_PREC_FP64 && c.precision != OCFD_PREC_MIXED)
+    return fail("precision must be OCFD_PREC_FP64 or OCFD_PREC_MIXED");
   for (int q = 0; q < 6; q++)
     if (c.scheme_boundary[q] < -1 || c.scheme_boundary[q] > 1) return fail("Scheme_boundary must be -1, 0 or 1");
   Block &B = h->B;
@@ -133,6 +134,10 @@ int set_params(ocfd_handle *h) {
   g.sx = c.nx + 2 * LAP; g.sy = c.ny + 2 * LAP; g.sz = c.nz + 2 * LAP;
   g.nh = (int64_t)g.sx * g.sy * g.sz;
   g.ni = (int64_t)g.nx * g.ny * g.nz;
+  // hybrid face lists hold 2 * face index + class in an int
+  if (c.scheme_invis == SCH_HYBRID &&
+      2 * (g.ni + std::max({(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny})) >= INT32_MAX)
+    return fail("HYBRID needs 2 * (faces per axis) < 2^31 on one block");
   // set_parameters (OCFD_readpara.f90:261-341)
   SchemeParams &S = B.S;
   S.scheme_invis = c.scheme_invis;

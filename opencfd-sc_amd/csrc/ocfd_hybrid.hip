@@ -5,8 +5,9 @@
 // WENO5, chosen per face from the sensor Rhybrid that comput_Rhybrid forms
 // once per time step.  Computed inside the sweep tiles, the few WENO faces
 // leave most lanes of a tile idle; here they are
-//   k_hyb_list    listed once per step, per sweep axis (wave-aggregated
-//                 appends), with a face -> slot map for the sweeps;
+//   k_hyb_list    listed once per step, per sweep axis (a range of the
+//                 list reserved per workgroup, faces in order inside it),
+//                 with a face -> slot map for the sweeps;
 //   k_weno_faces  computed densely before each stage's sweeps: batches of
 //                 listed faces per workgroup, split fluxes of the 8 stencil
 //                 points in LDS, then one (face, characteristic row) -- or
@@ -24,42 +25,65 @@ namespace {
 constexpr int NT = 256;
 constexpr int FB = 48;  // listed faces per workgroup batch
 
-// One thread per face of the axis: scheme index from the sensor, append
-// the WENO faces to the axis list, write the slot map.
+// A workgroup lists LPT * NT consecutive faces of the axis: scheme index from
+// the sensor, the WENO faces appended to the axis list in face order, and the
+// face -> slot map.  The list position is reserved with one atomic per
+// workgroup; inside it the slots follow the face index.
+constexpr int LPT = 16;  // faces per thread
 template <int AXIS>
 __global__ __launch_bounds__(NT) void k_hyb_list(const Block B) {
   typedef TileIndex<AXIS> TI;
+  __shared__ int soff[LPT][NT / 64];
+  __shared__ int sbase;
   const Geom &g = B.g;
   const SchemeParams &S = B.S;
   const int n = g.n(AXIS);
   const int64_t nfa = TI::nfaces(g);
-  const int64_t fid = (int64_t)blockIdx.x * NT + threadIdx.x;
-  bool weno = false;
-  int code = 0;
-  if (fid < nfa) {
-    int s, w, c;
-    TI::face_of(g, fid, s, w, c);
-    // set_hybrid_scheme: the larger sensor value of the two cells of the face
-    const double r0 = dmax2(B.rhyb[TI::hidx(g, s, w, c)], B.rhyb[TI::hidx(g, s + 1, w, c)]);
-    const int scm = hybrid_index(r0, S.hyb2, S.hyb3);
-    // the characteristic path closes non-periodic block edges with the
-    // non-reflecting face whatever the scheme index
-    const bool closure = S.is_char && ((S.bidx[AXIS][0] == 1 && S.sb[2 * AXIS] == 0 && s == 0) ||
-                                       (S.bidx[AXIS][1] == 1 && S.sb[2 * AXIS + 1] == 0 && s == n));
-    weno = scm >= 2 && !closure;
-    code = (int)(fid * 2 + (scm == 3 ? 1 : 0));
+  const int64_t f0 = (int64_t)blockIdx.x * (NT * LPT) + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned wbits = 0, hbits = 0;  // bit it: face f0 + it * NT is WENO / WENO7-class
+  for (int it = 0; it < LPT; it++) {
+    const int64_t fid = f0 + (int64_t)it * NT;
+    bool weno = false;
+    if (fid < nfa) {
+      int s, w, c;
+      TI::face_of(g, fid, s, w, c);
+      // set_hybrid_scheme: the larger sensor value of the two cells of the face
+      const double r0 = dmax2(B.rhyb[TI::hidx(g, s, w, c)], B.rhyb[TI::hidx(g, s + 1, w, c)]);
+      const int scm = hybrid_index(r0, S.hyb2, S.hyb3);
+      // the characteristic path closes non-periodic block edges with the
+      // non-reflecting face whatever the scheme index
+      const bool closure = S.is_char && ((S.bidx[AXIS][0] == 1 && S.sb[2 * AXIS] == 0 && s == 0) ||
+                                         (S.bidx[AXIS][1] == 1 && S.sb[2 * AXIS + 1] == 0 && s == n));
+      weno = scm >= 2 && !closure;
+      if (weno) wbits |= 1u << it;
+      if (scm == 3) hbits |= 1u << it;
+    }
+    const unsigned long long mask = __ballot(weno);
+    if (lane == 0) soff[it][wave] = __popcll(mask);
   }
-  const unsigned long long mask = __ballot(weno);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (mask) {
-    const int leader = __ffsll((long long)mask) - 1;
-    if (lane == leader) base = atomicAdd(&B.hcount[AXIS], __popcll(mask));
-    base = __shfl(base, leader);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int it = 0; it < LPT; it++)
+      for (int q = 0; q < NT / 64; q++) {
+        const int c = soff[it][q];
+        soff[it][q] = t;
+        t += c;
+      }
+    sbase = t ? atomicAdd(&B.hcount[AXIS], t) : 0;
   }
-  const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-  if (fid < nfa) B.hmap[AXIS][fid] = weno ? slot : -1;
-  if (weno) B.hlist[AXIS][slot] = code;
+  __syncthreads();
+  const int base = sbase;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int it = 0; it < LPT; it++) {
+    const int64_t fid = f0 + (int64_t)it * NT;
+    const bool weno = (wbits >> it) & 1u;
+    const unsigned long long mask = __ballot(weno);
+    const int slot = base + soff[it][wave] + __popcll(mask & below);
+    if (fid < nfa) B.hmap[AXIS][fid] = weno ? slot : -1;
+    if (weno) B.hlist[AXIS][slot] = (int)(fid * 2 + ((hbits >> it) & 1u));
+  }
 }
 
 // Fluxes of the listed faces of one axis.  A fixed grid strides over batches
@@ -227,9 +251,10 @@ hipError_t weno_axis(const Block &B, hipStream_t st) {
 
 hipError_t launch_hybrid_lists(const Block &B, hipStream_t st) {
   const Geom &g = B.g;
-  hipLaunchKernelGGL(k_hyb_list<0>, dim3((unsigned)((TileIndex<0>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
-  hipLaunchKernelGGL(k_hyb_list<1>, dim3((unsigned)((TileIndex<1>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
-  hipLaunchKernelGGL(k_hyb_list<2>, dim3((unsigned)((TileIndex<2>::nfaces(g) + NT - 1) / NT)), dim3(NT), 0, st, B);
+  constexpr int64_t FPB = NT * LPT;
+  hipLaunchKernelGGL(k_hyb_list<0>, dim3((unsigned)((TileIndex<0>::nfaces(g) + FPB - 1) / FPB)), dim3(NT), 0, st, B);
+  hipLaunchKernelGGL(k_hyb_list<1>, dim3((unsigned)((TileIndex<1>::nfaces(g) + FPB - 1) / FPB)), dim3(NT), 0, st, B);
+  hipLaunchKernelGGL(k_hyb_list<2>, dim3((unsigned)((TileIndex<2>::nfaces(g) + FPB - 1) / FPB)), dim3(NT), 0, st, B);
   return hipGetLastError();
 }
 

@@ -45,7 +45,8 @@ __device__ inline double dh(const Block &B, const double *a, int axis, int i, in
 }
 
 // derivative of an interior (no halo) field; periodic neighbours wrap
-__device__ inline double di(const Block &B, const double *a, int axis, int i, int j, int k) {
+template <typename R>
+__device__ inline R di(const Block &B, const R *a, int axis, int i, int j, int k) {
   const Geom &g = B.g;
   const int n = g.n(axis);
   const int64_t base = g.ii(i, j, k);
@@ -107,12 +108,12 @@ __device__ inline int wrap_index(int t, int n, int R) {
 // per-lane branch: with FAST (all axes periodic, uniform metrics) it is one
 // basic block, and the loads of plane k+1 issued at its top are waited for
 // only after the arithmetic of plane k.
-template <bool CD8, bool FAST>
+template <bool CD8, bool FAST, typename RT>
 __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg) {
   typedef PlaneTile<CD8> PT;
   constexpr int R = PT::R, LX = PT::LX, LY = PT::LY, NPL = LX * LY, Q = 2 * R + 1;
   constexpr int NRF = (NPL + NT - 1) / NT;
-  __shared__ double sT[2][4 * NPL];  // u, v, w, T of one plane: [4][LY][LX]
+  __shared__ RT sT[2][4 * NPL];  // u, v, w, T of one plane: [4][LY][LX]
   const Geom &g = B.g;
   const TileCols tc = tile_cols(g, kseg);
   const double *F[4] = {B.u, B.v, B.w, B.T};
@@ -131,15 +132,15 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
     toff[r] = (ii + LAP - 1) + g.sx * (jj + LAP - 1);
   }
   const int64_t plane = (int64_t)g.sx * g.sy;
-  double pre[4][NRF];
+  RT pre[4][NRF];
   auto fetch_tile = [&](int k) {
     const int64_t base = plane * (k + LAP - 1);
 #pragma unroll
     for (int r = 0; r < NRF; r++)
 #pragma unroll
-      for (int f = 0; f < 4; f++) pre[f][r] = F[f][base + toff[r]];
+      for (int f = 0; f < 4; f++) pre[f][r] = RT(F[f][base + toff[r]]);
   };
-  auto store_tile = [&](double *dst) {
+  auto store_tile = [&](RT *dst) {
 #pragma unroll
     for (int r = 0; r < NRF; r++) {
       // lanes past the tile rewrite its last point with the same value
@@ -150,19 +151,22 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
   };
 
   // own column: u, v, w, T at planes k-R .. k+R
-  double q[4][Q];
+  RT q[4][Q];
   const int64_t hcol = g.hi(ic, jc, 0);
 #pragma unroll
   for (int f = 0; f < 4; f++)
 #pragma unroll
-    for (int d = 0; d < Q; d++) q[f][d] = F[f][hcol + (int64_t)(tc.kb - R + d) * sk];
+    for (int d = 0; d < Q; d++) q[f][d] = RT(F[f][hcol + (int64_t)(tc.kb - R + d) * sk]);
   fetch_tile(tc.kb);
   store_tile(sT[0]);
   __syncthreads();
 
   const Metrics &M = B.M;
-  auto met = [&](int qm, int64_t h) { return FAST ? M.u[qm] : M.get(qm, h); };
-  const double Tsb = 110.4 / B.P.ref_amu_t0, TmpR = 1.0 / B.P.re;
+  auto met = [&](int qm, int64_t h) { return RT(FAST ? M.u[qm] : M.get(qm, h)); };
+  const RT Tsb = RT(110.4 / B.P.ref_amu_t0), TmpR = RT(1.0 / B.P.re);
+  const RT cp = RT(B.P.cp), pr = RT(B.P.pr);
+  RT *const ev = reinterpret_cast<RT *>(B.ev);
+  RT *const trash = reinterpret_cast<RT *>(B.trash);
   const bool perx = FAST || B.S.iperiodic[0] == 1, pery = FAST || B.S.iperiodic[1] == 1,
              perz = FAST || B.S.iperiodic[2] == 1;
   const int c0 = (tc.ty + R) * LX + tc.tx + R;  // the lane's own point in the tile
@@ -172,17 +176,17 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
     // next plane (the last iteration reloads its own; nothing reads it)
     const int kn = k < tc.ke ? k + 1 : k;
     fetch_tile(kn);
-    double qn[4];
+    RT qn[4];
 #pragma unroll
-    for (int f = 0; f < 4; f++) qn[f] = F[f][hcol + (int64_t)(kn + R) * sk];
+    for (int f = 0; f < 4; f++) qn[f] = RT(F[f][hcol + (int64_t)(kn + R) * sk]);
 
-    const double *cur = sT[(k - tc.kb) & 1];
+    const RT *cur = sT[(k - tc.kb) & 1];
     const int i = ic, j = jc;
     const int64_t h = g.hi(i, j, k);
-    double dx[4], dy[4], dz[4];
+    RT dx[4], dy[4], dz[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-      const double *cf = cur + f * NPL + c0;
+      const RT *cf = cur + f * NPL + c0;
       dx[f] = dxyz0_point([&](int d) { return cf[d]; }, i, g.nx, B.P.dc[0], CD8, perx);
       dy[f] = dxyz0_point([&](int d) { return cf[d * LX]; }, j, g.ny, B.P.dc[1], CD8, pery);
       dz[f] = dxyz0_point([&](int d) { return q[f][R + d]; }, k, g.nz, B.P.dc[2], CD8, perz);
@@ -190,46 +194,46 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
       if (FAST) __builtin_amdgcn_sched_barrier(0);
     }
     // comput_Amu (OCFD_viscous.f90:135-149)
-    const double Tc = q[3][R];
-    const double amu = TmpR * (1.0 + Tsb) * sqrt(Tc * Tc * Tc) / (Tsb + Tc);
+    const RT Tc = q[3][R];
+    const RT amu = TmpR * (RT(1.0) + Tsb) * sqrt(Tc * Tc * Tc) / (Tsb + Tc);
     // OCFD_viscous.f90:42-53
-    const double uk = dx[0], ui = dy[0], us = dz[0];
-    const double vk = dx[1], vi = dy[1], vs = dz[1];
-    const double wk = dx[2], wi = dy[2], ws = dz[2];
-    const double Tk = dx[3], Ti = dy[3], Ts = dz[3];
-    const double akx = met(M_AKX, h), aky = met(M_AKY, h), akz = met(M_AKZ, h);
-    const double aix = met(M_AIX, h), aiy = met(M_AIY, h), aiz = met(M_AIZ, h);
-    const double asx = met(M_ASX, h), asy = met(M_ASY, h), asz = met(M_ASZ, h);
+    const RT uk = dx[0], ui = dy[0], us = dz[0];
+    const RT vk = dx[1], vi = dy[1], vs = dz[1];
+    const RT wk = dx[2], wi = dy[2], ws = dz[2];
+    const RT Tk = dx[3], Ti = dy[3], Ts = dz[3];
+    const RT akx = met(M_AKX, h), aky = met(M_AKY, h), akz = met(M_AKZ, h);
+    const RT aix = met(M_AIX, h), aiy = met(M_AIY, h), aiz = met(M_AIZ, h);
+    const RT asx = met(M_ASX, h), asy = met(M_ASY, h), asz = met(M_ASZ, h);
     // OCFD_viscous.f90:60-111
-    const double ux = uk * akx + ui * aix + us * asx;
-    const double vx = vk * akx + vi * aix + vs * asx;
-    const double wx = wk * akx + wi * aix + ws * asx;
-    const double Tx = Tk * akx + Ti * aix + Ts * asx;
-    const double uy = uk * aky + ui * aiy + us * asy;
-    const double vy = vk * aky + vi * aiy + vs * asy;
-    const double wy = wk * aky + wi * aiy + ws * asy;
-    const double Ty = Tk * aky + Ti * aiy + Ts * asy;
-    const double uz = uk * akz + ui * aiz + us * asz;
-    const double vz = vk * akz + vi * aiz + vs * asz;
-    const double wz = wk * akz + wi * aiz + ws * asz;
-    const double Tz = Tk * akz + Ti * aiz + Ts * asz;
-    const double div = ux + vy + wz;
-    const double Amu1 = amu;
-    const double Amuk = B.P.cp * amu / B.P.pr;
-    const double tmp2_3 = 2.0 / 3.0;
-    const double s11 = (2.0 * ux - tmp2_3 * div) * Amu1;
-    const double s22 = (2.0 * vy - tmp2_3 * div) * Amu1;
-    const double s33 = (2.0 * wz - tmp2_3 * div) * Amu1;
-    const double s12 = (uy + vx) * Amu1, s13 = (uz + wx) * Amu1, s23 = (vz + wy) * Amu1;
-    const double uu = q[0][R], vv = q[1][R], ww = q[2][R];
-    const double E1 = uu * s11 + vv * s12 + ww * s13 + Amuk * Tx;
-    const double E2 = uu * s12 + vv * s22 + ww * s23 + Amuk * Ty;
-    const double E3 = uu * s13 + vv * s23 + ww * s33 + Amuk * Tz;
-    double *eb = tc.valid ? B.ev + g.ii(i, j, k) : B.trash + threadIdx.x;
+    const RT ux = uk * akx + ui * aix + us * asx;
+    const RT vx = vk * akx + vi * aix + vs * asx;
+    const RT wx = wk * akx + wi * aix + ws * asx;
+    const RT Tx = Tk * akx + Ti * aix + Ts * asx;
+    const RT uy = uk * aky + ui * aiy + us * asy;
+    const RT vy = vk * aky + vi * aiy + vs * asy;
+    const RT wy = wk * aky + wi * aiy + ws * asy;
+    const RT Ty = Tk * aky + Ti * aiy + Ts * asy;
+    const RT uz = uk * akz + ui * aiz + us * asz;
+    const RT vz = vk * akz + vi * aiz + vs * asz;
+    const RT wz = wk * akz + wi * aiz + ws * asz;
+    const RT Tz = Tk * akz + Ti * aiz + Ts * asz;
+    const RT div = ux + vy + wz;
+    const RT Amu1 = amu;
+    const RT Amuk = cp * amu / pr;
+    const RT tmp2_3 = RT(2.0 / 3.0);
+    const RT s11 = (RT(2.0) * ux - tmp2_3 * div) * Amu1;
+    const RT s22 = (RT(2.0) * vy - tmp2_3 * div) * Amu1;
+    const RT s33 = (RT(2.0) * wz - tmp2_3 * div) * Amu1;
+    const RT s12 = (uy + vx) * Amu1, s13 = (uz + wx) * Amu1, s23 = (vz + wy) * Amu1;
+    const RT uu = q[0][R], vv = q[1][R], ww = q[2][R];
+    const RT E1 = uu * s11 + vv * s12 + ww * s13 + Amuk * Tx;
+    const RT E2 = uu * s12 + vv * s22 + ww * s23 + Amuk * Ty;
+    const RT E3 = uu * s13 + vv * s23 + ww * s33 + Amuk * Tz;
+    RT *eb = tc.valid ? ev + g.ii(i, j, k) : trash + threadIdx.x;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const double n1 = met(M_AKX1 + 3 * a, h), n2 = met(M_AKX1 + 3 * a + 1, h), n3 = met(M_AKX1 + 3 * a + 2, h);
-      double *e = eb + (int64_t)(4 * a) * es;
+      const RT n1 = met(M_AKX1 + 3 * a, h), n2 = met(M_AKX1 + 3 * a + 1, h), n3 = met(M_AKX1 + 3 * a + 2, h);
+      RT *e = eb + (int64_t)(4 * a) * es;
       e[0] = n1 * s11 + n2 * s12 + n3 * s13;
       e[es] = n1 * s12 + n2 * s22 + n3 * s23;
       e[2 * es] = n1 * s13 + n2 * s23 + n3 * s33;
@@ -249,14 +253,17 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
 }
 
 // viscous divergence of component m (1..4) at cell t (OCFD_viscous.f90:114-129)
-__device__ inline double visc_div_m(const Block &B, int m, int i, int j, int k) {
+template <typename RT>
+__device__ inline RT visc_div_m(const Block &B, int m, int i, int j, int k) {
   const int64_t ni = B.g.ni;
-  const double *e1 = B.ev + (int64_t)(m - 1) * ni;
-  const double *e2 = B.ev + (int64_t)(4 + m - 1) * ni;
-  const double *e3 = B.ev + (int64_t)(8 + m - 1) * ni;
+  const RT *ev = reinterpret_cast<const RT *>(B.ev);
+  const RT *e1 = ev + (int64_t)(m - 1) * ni;
+  const RT *e2 = ev + (int64_t)(4 + m - 1) * ni;
+  const RT *e3 = ev + (int64_t)(8 + m - 1) * ni;
   return di(B, e1, 0, i, j, k) + di(B, e2, 1, i, j, k) + di(B, e3, 2, i, j, k);
 }
 
+template <typename RT>
 __global__ __launch_bounds__(NT) void k_visc_div(const Block B) {
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -266,8 +273,8 @@ __global__ __launch_bounds__(NT) void k_visc_div(const Block B) {
   const double ajac = B.M.get(M_AJAC, g.hi(i, j, k));
 #pragma unroll
   for (int m = 1; m <= 4; m++) {
-    const double s = visc_div_m(B, m, i, j, k);
-    B.du[t + (int64_t)m * g.ni] = B.du[t + (int64_t)m * g.ni] + s * ajac;
+    const RT s = visc_div_m<RT>(B, m, i, j, k);
+    B.du[t + (int64_t)m * g.ni] = B.du[t + (int64_t)m * g.ni] + double(s) * ajac;
   }
 }
 
@@ -297,7 +304,7 @@ __device__ inline void refresh_of(double cv, double d, double u, double v, doubl
 // one-sided closures never read either.  Cells with T <= 0 are collected
 // per lane in a bit mask over the segment and listed after the loop.  The
 // loop body is branch-free in the same way as k_visc_flux.
-template <bool VISC, bool CD8, bool FAST>
+template <bool VISC, bool CD8, bool FAST, typename RT>
 __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int krk, const int refresh,
                                                         const int kseg, int *neg) {
   typedef PlaneTile<CD8> PT;
@@ -305,7 +312,8 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
   constexpr int N1 = LX * TYT, N2 = TXT * LY;       // points per component
   constexpr int NE = VISC ? 4 * (N1 + N2) : 1;
   constexpr int NR1 = VISC ? (N1 + NT - 1) / NT : 1, NR2 = VISC ? (N2 + NT - 1) / NT : 1;
-  __shared__ double sE[VISC ? 2 : 1][NE];
+  __shared__ RT sE[VISC ? 2 : 1][NE];
+  const RT *const ev = reinterpret_cast<const RT *>(B.ev);
   const Geom &g = B.g;
   const TileCols tc = tile_cols(g, kseg);
   const int nz = g.nz;
@@ -338,9 +346,9 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
       o2[r] = (ii - 1) + g.nx * (jj - 1);
     }
   }
-  double pre1[4][NR1], pre2[4][NR2];
+  RT pre1[4][NR1], pre2[4][NR2];
   auto fetch_tile = [&](int k) {
-    const double *p1 = B.ev + skz * (k - 1), *p2 = B.ev + 4 * ni + skz * (k - 1);
+    const RT *p1 = ev + skz * (k - 1), *p2 = ev + 4 * ni + skz * (k - 1);
 #pragma unroll
     for (int r = 0; r < NR1; r++)
 #pragma unroll
@@ -350,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
 #pragma unroll
       for (int f = 0; f < 4; f++) pre2[f][r] = p2[f * ni + o2[r]];
   };
-  auto store_tile = [&](double *dst) {
+  auto store_tile = [&](RT *dst) {
 #pragma unroll
     for (int r = 0; r < NR1; r++) {
       const int e = threadIdx.x + r * NT < N1 ? threadIdx.x + r * NT : N1 - 1;
@@ -365,9 +373,9 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
     }
   };
 
-  const double *e3 = B.ev + (int64_t)8 * ni;
+  const RT *e3 = ev + (int64_t)8 * ni;
   const int64_t col = g.ii(ic, jc, 1);
-  double q[4][Q];  // Ev3(1:4) at planes k-R .. k+R
+  RT q[4][Q];  // Ev3(1:4) at planes k-R .. k+R
   if (VISC) {
 #pragma unroll
     for (int d = 0; d < Q; d++) {
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
       f0[m] = B.f[t + m * ni];
       if (krk != 1) fn0[m] = B.fn[t + m * ni];
     }
-    double qn[4];
+    RT qn[4];
     if (VISC) {
       __builtin_amdgcn_sched_barrier(0);
       const int kn = k < tc.ke ? k + 1 : k;
@@ -407,16 +415,16 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
       for (int m = 0; m < 4; m++) qn[m] = e3[(int64_t)m * ni + o];
     }
     const double ajac = FAST ? B.M.u[M_AJAC] : B.M.get(M_AJAC, h);
-    double vd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    RT vd[5] = {RT(0), RT(0), RT(0), RT(0), RT(0)};
     if (VISC) {
-      const double *cur = sE[(k - tc.kb) & 1];
+      const RT *cur = sE[(k - tc.kb) & 1];
 #pragma unroll
       for (int m = 1; m <= 4; m++) {
-        const double *c1 = cur + (m - 1) * N1 + cx;
-        const double *c2 = cur + (m - 1) * N2 + cy;
-        const double d1 = dxyz0_point([&](int d) { return c1[d]; }, i, g.nx, B.P.dc[0], CD8, perx);
-        const double d2 = dxyz0_point([&](int d) { return c2[d * TXT]; }, j, g.ny, B.P.dc[1], CD8, pery);
-        const double d3 = dxyz0_point([&](int d) { return q[m - 1][R + d]; }, k, nz, B.P.dc[2], CD8, perz);
+        const RT *c1 = cur + (m - 1) * N1 + cx;
+        const RT *c2 = cur + (m - 1) * N2 + cy;
+        const RT d1 = dxyz0_point([&](int d) { return c1[d]; }, i, g.nx, B.P.dc[0], CD8, perx);
+        const RT d2 = dxyz0_point([&](int d) { return c2[d * TXT]; }, j, g.ny, B.P.dc[1], CD8, pery);
+        const RT d3 = dxyz0_point([&](int d) { return q[m - 1][R + d]; }, k, nz, B.P.dc[2], CD8, perz);
         vd[m] = d1 + d2 + d3;
         if (FAST) __builtin_amdgcn_sched_barrier(0);  // one component's stencil at a time
       }
@@ -427,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       double dum = du[m];
-      if (VISC && m > 0) dum = dum + vd[m] * ajac;
+      if (VISC && m > 0) dum = dum + double(vd[m]) * ajac;
       const double f = f0[m];
       double r;
       if (krk == 1) {
@@ -496,34 +504,52 @@ static inline bool all_periodic(const Block &B) {
   return B.S.iperiodic[0] == 1 && B.S.iperiodic[1] == 1 && B.S.iperiodic[2] == 1 && B.M.uniform;
 }
 
+// Ev and the viscous derivatives are FP32 in the mixed-precision form; the
+// primitives they are formed from, du and the stage update stay FP64
+template <typename RT>
+static void visc_flux_as(const Block &B, dim3 grid, int kseg, hipStream_t st) {
+  const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
+  if (cd8 && per) hipLaunchKernelGGL((k_visc_flux<true, true, RT>), grid, dim3(NT), 0, st, B, kseg);
+  else if (cd8) hipLaunchKernelGGL((k_visc_flux<true, false, RT>), grid, dim3(NT), 0, st, B, kseg);
+  else if (per) hipLaunchKernelGGL((k_visc_flux<false, true, RT>), grid, dim3(NT), 0, st, B, kseg);
+  else hipLaunchKernelGGL((k_visc_flux<false, false, RT>), grid, dim3(NT), 0, st, B, kseg);
+}
+
 hipError_t launch_visc_flux(const Block &B, hipStream_t st) {
   const int kseg = march_kseg(B.g);
   const dim3 grid(march_blocks(B.g, kseg));
-  const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
-  if (cd8 && per) hipLaunchKernelGGL((k_visc_flux<true, true>), grid, dim3(NT), 0, st, B, kseg);
-  else if (cd8) hipLaunchKernelGGL((k_visc_flux<true, false>), grid, dim3(NT), 0, st, B, kseg);
-  else if (per) hipLaunchKernelGGL((k_visc_flux<false, true>), grid, dim3(NT), 0, st, B, kseg);
-  else hipLaunchKernelGGL((k_visc_flux<false, false>), grid, dim3(NT), 0, st, B, kseg);
+  if (B.prec == OCFD_PREC_MIXED_CODE) visc_flux_as<float>(B, grid, kseg, st);
+  else visc_flux_as<double>(B, grid, kseg, st);
   return hipGetLastError();
 }
 
 hipError_t launch_visc_div(const Block &B, hipStream_t st) {
-  hipLaunchKernelGGL(k_visc_div, dim3(nblocks(B.g.ni)), dim3(NT), 0, st, B);
+  if (B.prec == OCFD_PREC_MIXED_CODE) hipLaunchKernelGGL(k_visc_div<float>, dim3(nblocks(B.g.ni)), dim3(NT), 0, st, B);
+  else hipLaunchKernelGGL(k_visc_div<double>, dim3(nblocks(B.g.ni)), dim3(NT), 0, st, B);
   return hipGetLastError();
+}
+
+template <typename RT>
+static void visc_div_update_as(const Block &B, dim3 grid, int kseg, int krk, int refresh_planes, int *neg,
+                               hipStream_t st) {
+  const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
+#define OCFD_DIVUPD(V, C, P) \
+  hipLaunchKernelGGL((k_visc_div_update<V, C, P, RT>), grid, dim3(NT), 0, st, B, krk, refresh_planes, kseg, neg)
+  if (cd8 && per) OCFD_DIVUPD(true, true, true);
+  else if (cd8) OCFD_DIVUPD(true, true, false);
+  else if (per) OCFD_DIVUPD(true, false, true);
+  else OCFD_DIVUPD(true, false, false);
+#undef OCFD_DIVUPD
 }
 
 hipError_t launch_visc_div_update(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
   const int kseg = march_kseg(B.g);
   const dim3 grid(march_blocks(B.g, kseg));
-  const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
-#define OCFD_DIVUPD(V, C, P) \
-  hipLaunchKernelGGL((k_visc_div_update<V, C, P>), grid, dim3(NT), 0, st, B, krk, refresh_planes, kseg, neg)
-  if (B.ev == nullptr) OCFD_DIVUPD(false, false, false);
-  else if (cd8 && per) OCFD_DIVUPD(true, true, true);
-  else if (cd8) OCFD_DIVUPD(true, true, false);
-  else if (per) OCFD_DIVUPD(true, false, true);
-  else OCFD_DIVUPD(true, false, false);
-#undef OCFD_DIVUPD
+  if (B.ev == nullptr)
+    hipLaunchKernelGGL((k_visc_div_update<false, false, false, double>), grid, dim3(NT), 0, st, B, krk,
+                       refresh_planes, kseg, neg);
+  else if (B.prec == OCFD_PREC_MIXED_CODE) visc_div_update_as<float>(B, grid, kseg, krk, refresh_planes, neg, st);
+  else visc_div_update_as<double>(B, grid, kseg, krk, refresh_planes, neg, st);
   return hipGetLastError();
 }
 

@@ -79,8 +79,12 @@ def box_coords(nx, ny, nz):
     xs = 2.0 * PI / float(nx) * np.arange(1, nx + 1, dtype=np.float64)
     ys = 2.0 * PI / float(ny) * np.arange(1, ny + 1, dtype=np.float64)
     zs = 2.0 * PI / float(nz) * np.arange(1, nz + 1, dtype=np.float64)
-    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
-    return X.copy(), Y.copy(), Z.copy(), (xs, ys, zs)
+    shape = (nz, ny, nx)
+    # read-only broadcast views: consumers that need memory copy them
+    X = np.broadcast_to(xs[None, None, :], shape)
+    Y = np.broadcast_to(ys[None, :, None], shape)
+    Z = np.broadcast_to(zs[:, None, None], shape)
+    return X, Y, Z, (xs, ys, zs)
 
 
 def uniform_box_metrics(nx, ny, nz, lx=2 * PI, ly=2 * PI, lz=2 * PI):
@@ -135,15 +139,32 @@ def _keyed_normals(seed, kx, ky, kz, stream):
     return r * np.cos(2.0 * np.pi * u2), r * np.sin(2.0 * np.pi * u2)
 
 
-def solenoidal_field(n, seed, k0=8.0, kmax=None):
+def _ifftn_real(a):
+    """Real part of the inverse 3-D FFT, on all host cores."""
+    try:
+        import scipy.fft as sfft
+        return np.real(sfft.ifftn(a, workers=os.cpu_count() or 1))
+    except ImportError:
+        return np.real(np.fft.ifftn(a))
+
+
+def solenoidal_field(n, seed, k0=8.0, kmax=None, kcut=None):
     """Random solenoidal velocity on an n^3 periodic box with energy spectrum
-    E(k) ~ k^4 exp(-2 k^2 / k0^2); normalised to <|u|^2> = 1."""
-    k1 = np.fft.fftfreq(n, d=1.0 / n)
-    kz, ky, kx = np.meshgrid(k1, k1, k1, indexing="ij")
-    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    E(k) ~ k^4 exp(-2 k^2 / k0^2); normalised to <|u|^2> = 1.  Modes are drawn
+    for 0 < |k| <= min(kmax, kcut) (kmax = n/2 - 1; kcut = 6 k0, where the
+    amplitude has fallen to ~1e-15 of its peak), so the draw costs the same
+    at any n and a field on N^3 and on (2N)^3 share every drawn mode."""
     if kmax is None:
         kmax = n // 2 - 1
-    mask = (kk > 0) & (kk <= kmax)
+    if kcut is None:
+        kcut = 6.0 * k0
+    kc = min(float(kmax), float(kcut))
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    sel = np.nonzero(np.abs(k1) <= kc)[0]          # the sub-cube holding the sphere
+    ks = k1[sel]
+    kz, ky, kx = np.meshgrid(ks, ks, ks, indexing="ij")
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    mask = (kk > 0) & (kk <= kc)
     comps = []
     for c in range(3):
         re, im = _keyed_normals(seed, kx, ky, kz, c)
@@ -153,7 +174,12 @@ def solenoidal_field(n, seed, k0=8.0, kmax=None):
     kd = (kx * ax + ky * ay + kz * az) / k2
     ax, ay, az = ax - kx * kd, ay - ky * kd, az - kz * kd
     amp = np.where(mask, kk * np.exp(-kk * kk / (k0 * k0)), 0.0)
-    vel = [np.real(np.fft.ifftn(amp * a)) for a in (ax, ay, az)]
+    vel = []
+    for a in (ax, ay, az):
+        full = np.zeros((n, n, n), dtype=np.complex128)
+        full[np.ix_(sel, sel, sel)] = amp * a
+        vel.append(_ifftn_real(full))
+        del full
     e = np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
     s = 1.0 / np.sqrt(e)
     return [v * s for v in vel]
@@ -204,7 +230,13 @@ def isotropic(n=256, seed=20261202, k0=8.0, mt=0.5, re_lambda=72.0, cfl=0.5,
     ma = mt
     # Re_lambda = VV_rms^2 / Ux_rms * Re (mu(T=1) = 1/Re, rho = 1)
     kx = np.fft.fftfreq(n, d=1.0 / n)[None, None, :]
-    ux = np.real(np.fft.ifftn(1j * kx * np.fft.fftn(u)))
+    try:
+        import scipy.fft as sfft
+        uh = sfft.fftn(u, workers=os.cpu_count() or 1)
+    except ImportError:
+        uh = np.fft.fftn(u)
+    ux = _ifftn_real(1j * kx * uh)
+    del uh
     ux_rms = np.sqrt(np.mean(ux * ux))
     vv2 = np.mean(u * u + v * v + w * w) / 3.0
     re = re_lambda * ux_rms / vv2
@@ -376,16 +408,33 @@ def cfl_dt(case: Case, cfl: float) -> float:
             return (fp - fm) / 2.0 * (n - 1)
         return np.gradient(F, axis=a) * (n - 1)
 
+    hs = [1.0 / (cfg.nx - 1), 1.0 / (cfg.ny - 1), 1.0 / (cfg.nz - 1)]
+    if _separable(X, Y, Z):
+        # x = x(i), y = y(j), z = z(k): the metric tensor is diagonal
+        gx = 1.0 / deriv(X[:1, :1, :], 0, 0)
+        gy = 1.0 / deriv(Y[:1, :, :1], 1, 1)
+        gz = 1.0 / deriv(Z[:, :1, :1], 2, 2)
+        rate = ((np.abs(u * gx) + c * np.abs(gx)) / hs[0] + (np.abs(v * gy) + c * np.abs(gy)) / hs[1]
+                + (np.abs(w * gz) + c * np.abs(gz)) / hs[2])
+        return float(cfl / rate.max())
     J = np.empty(X.shape + (3, 3))
     for comp, F in enumerate((X, Y, Z)):
         for axis in range(3):
             J[..., comp, axis] = deriv(F, axis, comp)
     G = np.linalg.inv(J)          # G[..., axis, comp] = d xi_axis / d x_comp
     rate = np.zeros(X.shape)
-    hs = [1.0 / (cfg.nx - 1), 1.0 / (cfg.ny - 1), 1.0 / (cfg.nz - 1)]
     for axis in range(3):
         gx, gy, gz = G[..., axis, 0], G[..., axis, 1], G[..., axis, 2]
         un = np.abs(u * gx + v * gy + w * gz)
         gn = np.sqrt(gx * gx + gy * gy + gz * gz)
         rate += (un + c * gn) / hs[axis]
     return float(cfl / rate.max())
+
+
+def _separable(X, Y, Z):
+    if all(a.base is not None and 0 in a.strides for a in (X, Y, Z)) and \
+            X.strides[:2] == (0, 0) and Y.strides[::2] == (0, 0) and Z.strides[1:] == (0, 0):
+        return True
+    return (np.array_equal(X, np.broadcast_to(X[:1, :1, :], X.shape))
+            and np.array_equal(Y, np.broadcast_to(Y[:1, :, :1], Y.shape))
+            and np.array_equal(Z, np.broadcast_to(Z[:, :1, :1], Z.shape)))

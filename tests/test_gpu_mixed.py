@@ -1,0 +1,98 @@
+"""Mixed-precision form (OCFD_PREC_MIXED) against the FP64 path.
+
+The mixed form forms the flux splitting, the reconstructions of the sweeps and
+the viscous derivatives / stresses / fluxes in FP32; primitives, f, fn, du, the
+RK3 stage combination and the metrics stay FP64, and so do the faces the
+hybrid sensor flags (characteristic projection and WENO).  Its results are
+compared with the FP64 path, which the other GPU tests pin bit for bit to the
+compiled reference and the oracle, and with the oracle directly.
+
+Tolerances, relative L2 per field:
+  whole steps of periodic isotropic turbulence (Mt = 0.5, the mixed form's
+    target case): <= 1e-5 after 20 steps (the accuracy bound the mixed form
+    is specified to), kinetic energy and enstrophy <= 1e-6 relative;
+  one right-hand side du on random blocks with block-edge closures and
+    curvilinear metrics: <= 2e-4 -- du is a difference of neighbouring face
+    fluxes, so FP32 rounding of the fluxes (relative 6e-8) is amplified by
+    |flux| / (|du| h), up to a few thousand on these blocks.
+Each comparison also requires a nonzero difference, i.e. that the FP32
+kernels ran.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from ocfd import cases, native
+from ocfd.config import PREC_MIXED
+from util import interior, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def block(case, precision=0, uniform=None):
+    cfg = case.cfg.replace(precision=precision)
+    return native.setup_block(native.Block(cfg), case, use_uniform=uniform)
+
+
+def rhs(b, case):
+    b.save_fn()
+    b.exchange_primitives()
+    if case.cfg.scheme_invis == "HYBRID":
+        b.comput_rhybrid()
+    (b.du_inviscous_character if case.cfg.if_scheme_character else b.du_inviscous)()
+    if case.cfg.if_viscous == 1:
+        b.du_viscous()
+    return b.download_du()
+
+
+RHS = [
+    dict(seed=71, scheme="WENO5", split=1, char=0),
+    dict(seed=72, scheme="WENO7", split=2, char=1),
+    dict(seed=73, scheme="HYBRID", split=2, char=1, jump=0.5, hybrid=(0.002, 0.02)),
+    dict(seed=74, scheme="OMP6", split=1, char=1, noise=0.03),
+    dict(seed=75, scheme="UD7L", split=2, char=0, vis="CD8"),
+    dict(seed=76, scheme="WENO5", split=1, char=1, grid="curv", iperiodic=(1, 0, 1),
+         scheme_boundary=(0, 0, 1, 0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("kw", RHS, ids=[m["scheme"] + str(m["seed"]) for m in RHS])
+def test_mixed_rhs_close_to_fp64(kw):
+    case = cases.random_case(shape=(20, 28, 40), **kw)
+    with block(case) as b64, block(case, PREC_MIXED) as bmx:
+        d64, dmx = rhs(b64, case), rhs(bmx, case)
+    for m in range(5):
+        err = rel_l2(dmx[m], d64[m])
+        print("du(%d) relative L2 %.3e" % (m + 1, err))
+        assert err <= 2e-4, "du(%d): %.3e" % (m + 1, err)
+    assert np.any(dmx != d64), "mixed du equals FP64 du bit for bit: FP32 kernels did not run"
+
+
+@pytest.mark.parametrize("scheme", ["HYBRID", "WENO7"])
+def test_mixed_isotropic_steps_close_to_fp64(scheme):
+    case = cases.isotropic(64, scheme=scheme, split=2, char=1, hybrid_thresholds=(0.01, 0.05), metric_mode=1)
+    with block(case, uniform=True) as b64, block(case, PREC_MIXED, uniform=True) as bmx:
+        for _ in range(4):
+            b64.step(5)
+            bmx.step(5)
+            a64, amx = b64.ana_box(), bmx.ana_box()
+            for q, nm in ((12, "kinetic energy"), (13, "enstrophy")):
+                assert abs(amx[q] - a64[q]) <= 1e-6 * abs(a64[q]), (nm, amx[q], a64[q])
+        p64, pmx = b64.download_primitives(), bmx.download_primitives()
+    errs = [rel_l2(interior(pmx[q]), interior(p64[q])) for q in range(5)]
+    print(scheme, "relative L2 after 20 steps (d u v w T):", " ".join("%.2e" % e for e in errs))
+    assert max(errs) <= 1e-5, errs
+    assert max(errs) > 0.0, "mixed run equals FP64 run bit for bit: FP32 kernels did not run"
+
+
+def test_mixed_against_oracle():
+    """The same bound against the oracle itself (non-periodic closures, field metrics)."""
+    case = cases.random_case(shape=(20, 28, 40), seed=77, scheme="WENO7", split=2, char=1)
+    ob = pyoracle.setup_oracle(case)
+    with block(case, PREC_MIXED) as bmx:
+        ob.step(3)
+        bmx.step(3)
+        pg, po = bmx.download_primitives(), ob.download_primitives()
+    errs = [rel_l2(interior(pg[q]), interior(po[q])) for q in range(5)]
+    print("relative L2 vs oracle after 3 steps:", " ".join("%.2e" % e for e in errs))
+    assert max(errs) <= 1e-5, errs
