@@ -513,13 +513,29 @@ int ocfd_download_fn(ocfd_handle *h, double *f) {
   HIPCHK(hipMemcpy(f, h->B.fn, h->f_bytes(), hipMemcpyDeviceToHost));
   return 0;
 }
+// du is FP32 on the device in the mixed-precision form: converted on the host
 int ocfd_download_du(ocfd_handle *h, double *f) {
   HIPCHK(hipStreamSynchronize(h->st));
-  HIPCHK(hipMemcpy(f, h->B.du, h->f_bytes(), hipMemcpyDeviceToHost));
+  if (h->B.prec != OCFD_PREC_MIXED_CODE) {
+    HIPCHK(hipMemcpy(f, h->B.du, h->f_bytes(), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  const size_t n = (size_t)5 * h->B.g.ni;
+  std::vector<float> tmp(n);
+  HIPCHK(hipMemcpy(tmp.data(), h->B.du, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < n; q++) f[q] = (double)tmp[q];
   return 0;
 }
 int ocfd_upload_du(ocfd_handle *h, const double *f) {
-  HIPCHK(hipMemcpy(h->B.du, f, h->f_bytes(), hipMemcpyHostToDevice));
+  if (h->B.prec != OCFD_PREC_MIXED_CODE) {
+    HIPCHK(hipMemcpy(h->B.du, f, h->f_bytes(), hipMemcpyHostToDevice));
+    return 0;
+  }
+  const size_t n = (size_t)5 * h->B.g.ni;
+  std::vector<float> tmp(n);
+  for (size_t q = 0; q < n; q++) tmp[q] = (float)f[q];
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(h->B.du, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
   return 0;
 }
 int ocfd_download_primitives(ocfd_handle *h, double *d, double *u, double *v, double *w, double *T) {

@@ -176,17 +176,21 @@ __global__ __launch_bounds__(NT) void k_hyb_counts(const Block B, unsigned long 
 // ---------------------------------------------------------------------------
 // RK3 stage (OCFD_Time_Adv.f90:4-36) / fn = f / init3d f / comput_duvwT
 // ---------------------------------------------------------------------------
+// du is held in DU (double; float in the mixed-precision form); the stage
+// combination is FP64
+template <typename DU>
 __global__ __launch_bounds__(NT) void k_rk3(const Block B, const int krk) {
   const int64_t n5 = 5 * B.g.ni;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= n5) return;
   const double dt = B.P.dt;
+  const double du = double(reinterpret_cast<const DU *>(B.du)[t]);
   if (krk == 1) {
-    B.f[t] = B.fn[t] + dt * B.du[t];
+    B.f[t] = B.fn[t] + dt * du;
   } else {
     const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
     const double rb = krk == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
-    B.f[t] = ra * B.fn[t] + rb * (B.f[t] + dt * B.du[t]);
+    B.f[t] = ra * B.fn[t] + rb * (B.f[t] + dt * du);
   }
 }
 
@@ -697,7 +701,8 @@ hipError_t launch_hybrid_counts(const Block &B, unsigned long long *counts, hipS
 }
 
 hipError_t launch_rk3(const Block &B, int krk, hipStream_t st) {
-  hipLaunchKernelGGL(k_rk3, dim3(nb(5 * B.g.ni)), dim3(NT), 0, st, B, krk);
+  if (B.prec == OCFD_PREC_MIXED_CODE) hipLaunchKernelGGL(k_rk3<float>, dim3(nb(5 * B.g.ni)), dim3(NT), 0, st, B, krk);
+  else hipLaunchKernelGGL(k_rk3<double>, dim3(nb(5 * B.g.ni)), dim3(NT), 0, st, B, krk);
   return hipGetLastError();
 }
 

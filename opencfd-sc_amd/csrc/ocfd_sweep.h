@@ -99,7 +99,9 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   }
   // du entries of the tile (accumulating modes), fetched before the face
   // fluxes are computed
-  double pdu[NI3];
+  // du is stored in the sweep's real type (FP32 in the mixed-precision form)
+  R *const du = reinterpret_cast<R *>(B.du);
+  R pdu[NI3];
 
   R *sF = reinterpret_cast<R *>(smem);             // fp(5), fm(5)      [10][NP][TW]
   // the characteristic inputs are needed only where the sweep itself
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
     for (int r = 0; r < NI3; r++) {
       int sl_, wl, m;
       if (!p3(r, sl_, wl, m)) continue;
-      pdu[r] = B.du[TI::iidx(g, s0 + sl_, w0 + wl, c) + (int64_t)m * g.ni];
+      pdu[r] = du[TI::iidx(g, s0 + sl_, w0 + wl, c) + (int64_t)m * g.ni];
     }
   }
 
@@ -338,19 +340,19 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
     }
     const int64_t q = TI::iidx(g, s, w, c) + (int64_t)m * g.ni;
     if (mode == INV_ASSIGN) {
-      B.du[q] = double(val);
+      du[q] = val;
     } else if (mode == INV_ACCUM) {
-      B.du[q] = pdu[r] + double(val);
+      du[q] = pdu[r] + val;
     } else {
       const int64_t h = TI::hidx(g, s, w, c);
-      double rr = -(pdu[r] + double(val)) * (FAST ? B.M.u[M_AJAC] : B.M.get(M_AJAC, h));
+      R rr = -(pdu[r] + val) * R(FAST ? B.M.u[M_AJAC] : B.M.get(M_AJAC, h));
       if (B.P.if_mass_force == 1 && m > 0) {
         // OCFD_inviscous.f90:189-201
         const double *mf = B.P.mass_force;
-        if (m < 4) rr = rr + B.d[h] * mf[m - 1];
-        else rr = rr + B.d[h] * (B.u[h] * mf[0] + B.v[h] * mf[1] + B.w[h] * mf[2]);
+        if (m < 4) rr = rr + R(B.d[h] * mf[m - 1]);
+        else rr = rr + R(B.d[h] * (B.u[h] * mf[0] + B.v[h] * mf[1] + B.w[h] * mf[2]));
       }
-      B.du[q] = rr;
+      du[q] = rr;
     }
   }
 }

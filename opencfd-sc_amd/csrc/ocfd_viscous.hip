@@ -274,7 +274,8 @@ __global__ __launch_bounds__(NT) void k_visc_div(const Block B) {
 #pragma unroll
   for (int m = 1; m <= 4; m++) {
     const RT s = visc_div_m<RT>(B, m, i, j, k);
-    B.du[t + (int64_t)m * g.ni] = B.du[t + (int64_t)m * g.ni] + double(s) * ajac;
+    RT *du = reinterpret_cast<RT *>(B.du) + t + (int64_t)m * g.ni;
+    du[0] = RT(double(du[0]) + double(s) * ajac);
   }
 }
 
@@ -314,6 +315,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
   constexpr int NR1 = VISC ? (N1 + NT - 1) / NT : 1, NR2 = VISC ? (N2 + NT - 1) / NT : 1;
   __shared__ RT sE[VISC ? 2 : 1][NE];
   const RT *const ev = reinterpret_cast<const RT *>(B.ev);
+  const RT *const dus = reinterpret_cast<const RT *>(B.du);  // du in the right-hand side's real type
   const Geom &g = B.g;
   const TileCols tc = tile_cols(g, kseg);
   const int nz = g.nz;
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
     double du[5], f0[5], fn0[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-      du[m] = B.du[t + m * ni];
+      du[m] = double(dus[t + m * ni]);
       f0[m] = B.f[t + m * ni];
       if (krk != 1) fn0[m] = B.fn[t + m * ni];
     }
@@ -545,7 +547,10 @@ static void visc_div_update_as(const Block &B, dim3 grid, int kseg, int krk, int
 hipError_t launch_visc_div_update(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
   const int kseg = march_kseg(B.g);
   const dim3 grid(march_blocks(B.g, kseg));
-  if (B.ev == nullptr)
+  if (B.ev == nullptr && B.prec == OCFD_PREC_MIXED_CODE)
+    hipLaunchKernelGGL((k_visc_div_update<false, false, false, float>), grid, dim3(NT), 0, st, B, krk,
+                       refresh_planes, kseg, neg);
+  else if (B.ev == nullptr)
     hipLaunchKernelGGL((k_visc_div_update<false, false, false, double>), grid, dim3(NT), 0, st, B, krk,
                        refresh_planes, kseg, neg);
   else if (B.prec == OCFD_PREC_MIXED_CODE) visc_div_update_as<float>(B, grid, kseg, krk, refresh_planes, neg, st);
