@@ -172,6 +172,18 @@ int ocfd_advance_clock(ocfd_handle *h);
  * three RK stages, Istep/tt update and filtering.  State stays on device. */
 int ocfd_step(ocfd_handle *h, int32_t nsteps);
 
+/* ---- sweep tiles (no counterpart in the reference) --------------------- */
+/* Tile shape of the inviscid sweep of each axis: 0 = x 64x4, y/z 32x8 points
+ * x lines (the default); 1 = x 32x8, y/z 16x16; 2 = x 128x2, y/z 64x4.
+ * Shapes 1 and 2 apply to periodic axes with OCFD_METRIC_UNIFORM; other axes
+ * always use 0.  Results are bit-identical for every shape. */
+int ocfd_set_sweep_tiles(ocfd_handle *h, const int32_t shape[3]);
+int ocfd_get_sweep_tiles(ocfd_handle *h, int32_t shape[3]);
+/* Autotuner: times `reps` launches of each axis' sweep in every shape it
+ * supports on the block's current state and keeps the fastest (shape_out may
+ * be NULL).  Overwrites du (a work array): call it between time steps. */
+int ocfd_autotune(ocfd_handle *h, int32_t reps, int32_t shape_out[3]);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* show_flow_msg sums (OCFD_Time_Adv.f90:65-90): out[0..2] = E, K, S
  * (averaged over the global cell count). */

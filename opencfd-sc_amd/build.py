@@ -4,8 +4,9 @@
     python opencfd-sc_amd/build.py --clean
 
 FP64 kernels are compiled with -ffp-contract=off so that every floating-point
-expression rounds exactly as the reference's (no fused multiply-add), and with
-packed-FP32 instructions disabled.
+expression rounds exactly as the reference's (no fused multiply-add); the FP32
+kernels of the mixed-precision form (CONTRACT) allow contraction.  Packed-FP32
+instructions are disabled everywhere.
 """
 from __future__ import annotations
 
@@ -22,7 +23,10 @@ LIB = os.path.join(HERE, "libocfd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ocfd_inviscid.hip", "ocfd_inviscid_mixed.hip", "ocfd_hybrid.hip", "ocfd_viscous.hip", "ocfd_misc.hip", "ocfd_api.hip"]
+SOURCES = ["ocfd_inviscid.hip", "ocfd_inviscid_mixed.hip", "ocfd_hybrid.hip", "ocfd_viscous.hip",
+           "ocfd_viscous_mixed.hip", "ocfd_misc.hip", "ocfd_api.hip"]
+# FP32 kernels of the mixed-precision form: FMA contraction allowed
+CONTRACT = {"ocfd_inviscid_mixed.hip", "ocfd_viscous_mixed.hip"}
 FLAGS = [
     "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -33,7 +37,8 @@ FLAGS = [
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    flags = [("-ffp-contract=fast" if (f == "-ffp-contract=off" and src in CONTRACT) else f) for f in FLAGS]
+    cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -51,6 +51,7 @@ struct ocfd_handle {
   double tt = 0.0;
   bool prof = false;
   bool use_graph = true;           // replay whole steps from a captured hipGraph
+  int tile[3] = {0, 0, 0};         // sweep tile shape per axis (SweepTile)
   hipGraphExec_t gexec = nullptr;  // one time step (three stages), captured on first use
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
@@ -234,7 +235,8 @@ int inviscid(ocfd_handle *h, bool ch) {
                                  {"inviscid_char_x", "inviscid_char_y", "inviscid_char_z"}};
   const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
   for (int a = 0; a < 3; a++)
-    if (launch(h, nm[ch][a], [&] { return launch_inviscid(B, a, modes[a], SweepTile{0}, h->st); })) return -1;
+    if (launch(h, nm[ch][a], [&] { return launch_inviscid(B, a, modes[a], SweepTile{h->tile[a]}, h->st); }))
+      return -1;
   return 0;
 }
 
@@ -687,6 +689,73 @@ int ocfd_step(ocfd_handle *h, int32_t nsteps) {
   if (read_neg(h, cnt)) return -1;
   if (cnt[2] > 10) return fail("more than 10 cells with T <= 0 in one stage");
   return cnt[1];
+}
+
+// ---- sweep tiles ---------------------------------------------------------------
+int ocfd_set_sweep_tiles(ocfd_handle *h, const int32_t shape[3]) {
+  for (int a = 0; a < 3; a++)
+    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0, 1 or 2");
+  for (int a = 0; a < 3; a++) h->tile[a] = shape[a];
+  invalidate_graph(h);
+  return 0;
+}
+
+int ocfd_get_sweep_tiles(ocfd_handle *h, int32_t shape[3]) {
+  for (int a = 0; a < 3; a++) shape[a] = h->tile[a];
+  return 0;
+}
+
+int ocfd_autotune(ocfd_handle *h, int32_t reps, int32_t shape_out[3]) {
+  if (reps < 1) reps = 5;
+  const ocfd_config &c = h->cfg;
+  const bool ch = c.if_scheme_character != 0;
+  const bool prof = h->prof;
+  h->prof = false;
+  // the state the sweeps see in a stage: halos, and for the hybrid scheme the
+  // sensor, the face lists and the WENO faces
+  int rc = exchange_prims(h);
+  if (!rc && c.scheme_invis == SCH_HYBRID) rc = sensor(h);
+  if (!rc) rc = inviscid(h, ch);
+  h->prof = prof;
+  if (rc) return -1;
+  Block B = h->B;
+  B.S.is_char = ch ? 1 : 0;
+  const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  rc = 0;
+  for (int a = 0; a < 3 && !rc; a++) {
+    // the alternative shapes exist for periodic axes with uniform metrics
+    const int nshape = (c.iperiodic[a] == 1 && B.M.uniform) ? NSWEEP_SHAPES : 1;
+    float best = 0.f;
+    int bests = 0;
+    for (int sh = 0; sh < nshape && !rc; sh++) {
+      hipError_t e = launch_inviscid(B, a, modes[a], SweepTile{sh}, h->st);  // warm-up
+      e = e != hipSuccess ? e : hipEventRecord(e0, h->st);
+      for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_inviscid(B, a, modes[a], SweepTile{sh}, h->st);
+      e = e != hipSuccess ? e : hipEventRecord(e1, h->st);
+      e = e != hipSuccess ? e : hipEventSynchronize(e1);
+      float ms = 0.f;
+      e = e != hipSuccess ? e : hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      if (sh == 0 || ms < best) {
+        best = ms;
+        bests = sh;
+      }
+    }
+    h->tile[a] = bests;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return -1;
+  invalidate_graph(h);
+  if (shape_out)
+    for (int a = 0; a < 3; a++) shape_out[a] = h->tile[a];
+  return 0;
 }
 
 // ---- diagnostics -------------------------------------------------------------

@@ -24,6 +24,18 @@ struct Block {
   int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  xcd_tile
+// turns the dispatch index into a tile index such that every XCD works
+// through one contiguous range of tiles, so neighbouring tiles, which read
+// each other's halo points, share an L2.  A bijection on [0, nb).
+constexpr int NXCD = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int x = b % NXCD, i = b / NXCD;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // Index arithmetic of a sweep along AXIS: point s of line w, third index c
 // (x: lines j, third k; y: lines i, third k; z: lines i, third j).
 template <int AXIS>
@@ -67,7 +79,11 @@ struct TileIndex {
 
 enum InvMode : int { INV_ASSIGN = 0, INV_ACCUM = 1, INV_FINAL = 2 };
 
-// Launch shape of one sweep (0: the built-in tile of each axis).
+// Launch shape of one sweep: 0 is the built-in tile of each axis (x 64x4,
+// y and z 32x8 points x lines); 1 (x 32x8, y/z 16x16) and 2 (x 128x2, y/z
+// 64x4) exist for periodic axes with uniform metrics and are chosen by the
+// autotuner (ocfd_autotune).
+constexpr int NSWEEP_SHAPES = 3;
 struct SweepTile {
   int shape;
 };

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   const int n = g.n(AXIS);
   const int nw = TI::nwidth(g), nc = TI::nthird(g);
   const int nts = (n + TL - 1) / TL, ntw = (nw + TW - 1) / TW;
-  int b = blockIdx.x;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int ts = b % nts; b /= nts;
   const int tw = b % ntw;
   const int c = b / ntw + 1;
@@ -378,10 +378,19 @@ hipError_t launch_one(const Block &B, int mode, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int AXIS, bool CHAR, bool FAST, typename R>
+// Tile shapes of a sweep (TL points along the axis x TW lines, 256 threads).
+// Shape 0 is the default; shapes 1 and 2 are the alternatives the autotuner
+// times, built for the periodic uniform-metric kernels only.
+//   x: 64x4, 32x8, 128x2        y, z: 32x8, 16x16, 64x4
+template <int AXIS>
+constexpr int shape_tl(int shape) {
+  return AXIS == 0 ? (shape == 1 ? 32 : (shape == 2 ? 128 : 64)) : (shape == 1 ? 16 : (shape == 2 ? 64 : 32));
+}
+
+template <int AXIS, bool CHAR, bool FAST, typename R, int SHAPE>
 hipError_t dispatch_scheme(const Block &B, int mode, hipStream_t st) {
-  constexpr int TL = AXIS == 0 ? 64 : 32;
-  constexpr int TW = AXIS == 0 ? 4 : 8;
+  constexpr int TL = shape_tl<AXIS>(SHAPE);
+  constexpr int TW = NTHREADS / TL;
   switch (B.S.scheme_invis) {
     case SCH_WENO5: return launch_one<AXIS, SCH_WENO5, CHAR, TL, TW, FAST, R>(B, mode, st);
     case SCH_WENO7: return launch_one<AXIS, SCH_WENO7, CHAR, TL, TW, FAST, R>(B, mode, st);
@@ -393,20 +402,30 @@ hipError_t dispatch_scheme(const Block &B, int mode, hipStream_t st) {
 }
 
 template <int AXIS, typename R>
-hipError_t dispatch_axis(const Block &B, int mode, hipStream_t st) {
+hipError_t dispatch_axis(const Block &B, int mode, int shape, hipStream_t st) {
   const bool ch = B.S.is_char != 0;
   const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
-  if (fast)
-    return ch ? dispatch_scheme<AXIS, true, true, R>(B, mode, st) : dispatch_scheme<AXIS, false, true, R>(B, mode, st);
-  return ch ? dispatch_scheme<AXIS, true, false, R>(B, mode, st) : dispatch_scheme<AXIS, false, false, R>(B, mode, st);
+  if (fast) {
+    if (shape == 1)
+      return ch ? dispatch_scheme<AXIS, true, true, R, 1>(B, mode, st)
+                : dispatch_scheme<AXIS, false, true, R, 1>(B, mode, st);
+    if (shape == 2)
+      return ch ? dispatch_scheme<AXIS, true, true, R, 2>(B, mode, st)
+                : dispatch_scheme<AXIS, false, true, R, 2>(B, mode, st);
+    return ch ? dispatch_scheme<AXIS, true, true, R, 0>(B, mode, st)
+              : dispatch_scheme<AXIS, false, true, R, 0>(B, mode, st);
+  }
+  return ch ? dispatch_scheme<AXIS, true, false, R, 0>(B, mode, st)
+            : dispatch_scheme<AXIS, false, false, R, 0>(B, mode, st);
 }
 
-// the three axes of one real type
+// the three axes of one real type; shape selects the tile (periodic
+// uniform-metric sweeps only; the others always use shape 0)
 template <typename R>
-hipError_t launch_sweep(const Block &B, int axis, int mode, hipStream_t st) {
-  if (axis == 0) return dispatch_axis<0, R>(B, mode, st);
-  if (axis == 1) return dispatch_axis<1, R>(B, mode, st);
-  return dispatch_axis<2, R>(B, mode, st);
+hipError_t launch_sweep(const Block &B, int axis, int mode, int shape, hipStream_t st) {
+  if (axis == 0) return dispatch_axis<0, R>(B, mode, shape, st);
+  if (axis == 1) return dispatch_axis<1, R>(B, mode, shape, st);
+  return dispatch_axis<2, R>(B, mode, shape, st);
 }
 
 }  // namespace

@@ -1,0 +1,18 @@
+// ocfd_viscous_mixed.hip -- FP32 instantiations of the viscous kernels for
+// the mixed-precision form (Ev, the derivatives, stresses and fluxes in FP32;
+// du accumulated in FP32; the stage combination and primitives FP64).  Built
+// with FMA contraction allowed: this form is not bit-matched to the
+// reference.
+#include "ocfd_visc.h"
+
+namespace ocfd {
+
+hipError_t launch_visc_flux_mixed(const Block &B, hipStream_t st) { return visc_flux_as<float>(B, st); }
+
+hipError_t launch_visc_div_mixed(const Block &B, hipStream_t st) { return visc_div_as<float>(B, st); }
+
+hipError_t launch_visc_div_update_mixed(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
+  return visc_div_update_as<float>(B, krk, refresh_planes, neg, st);
+}
+
+}  // namespace ocfd

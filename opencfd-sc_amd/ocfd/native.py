@@ -251,6 +251,23 @@ class Block(BlockBase):
     def synchronize(self):
         return self._call("synchronize")
 
+    # -- sweep tiles ------------------------------------------------------------
+    def set_sweep_tiles(self, shapes):
+        a = (ctypes.c_int32 * 3)(*[int(x) for x in shapes])
+        return self._call("set_sweep_tiles", a)
+
+    def get_sweep_tiles(self):
+        a = (ctypes.c_int32 * 3)()
+        self._call("get_sweep_tiles", a)
+        return list(a)
+
+    def autotune(self, reps=5):
+        """Time every tile shape of each axis' sweep on the current state and
+        keep the fastest; returns the chosen shapes.  Overwrites du."""
+        a = (ctypes.c_int32 * 3)()
+        self._call("autotune", ctypes.c_int32(reps), a)
+        return list(a)
+
     def profile_enable(self, on=True):
         return self._call("profile_enable", ctypes.c_int32(1 if on else 0))
 

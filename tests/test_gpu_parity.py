@@ -238,3 +238,45 @@ def test_tgv64_100_steps_against_oracle():
             assert rel_l2(interior(pg[q]), interior(po[q])) <= 1e-12, nm
         np.testing.assert_allclose(ke_g, ke_o, rtol=1e-10)
         np.testing.assert_allclose(en_g, en_o, rtol=1e-10)
+
+
+def _rhs(blk, case):
+    blk.save_fn()
+    blk.exchange_primitives()
+    if case.cfg.scheme_invis == "HYBRID":
+        blk.comput_rhybrid()
+    (blk.du_inviscous_character if case.cfg.if_scheme_character else blk.du_inviscous)()
+    blk.du_viscous()
+    return blk.download_du()
+
+
+@pytest.mark.parametrize("kw", UNIFORM[:5], ids=[m["scheme"] + str(m["seed"]) for m in UNIFORM[:5]])
+def test_sweep_tile_shapes_bitwise(kw):
+    """Every sweep tile shape (the autotuner's choices) gives the same bits:
+    one right-hand side and two whole steps, against shape 0."""
+    case = cases.random_case(shape=(20, 28, 40), **kw)
+    case.uniform_metrics = cases.uniform_box_metrics(case.cfg.nx, case.cfg.ny, case.cfg.nz)
+    case.cfg.metric_mode = 1
+    with gpu_block(case, uniform=True) as ref:
+        du0 = _rhs(ref, case)
+        ref.step(2)
+        f0 = ref.download_f()
+    for shapes in ([1, 1, 1], [2, 2, 2], [1, 2, 0]):
+        with gpu_block(case, uniform=True) as b:
+            b.set_sweep_tiles(shapes)
+            assert b.get_sweep_tiles() == shapes
+            assert_same(_rhs(b, case), du0, "du with tiles %s" % shapes)
+            b.step(2)
+            assert_same(b.download_f(), f0, "f after 2 steps with tiles %s" % shapes)
+
+
+def test_autotune_keeps_results():
+    case = cases.random_case(shape=(20, 28, 40), seed=57, scheme="WENO7", split=2, char=1)
+    case.uniform_metrics = cases.uniform_box_metrics(case.cfg.nx, case.cfg.ny, case.cfg.nz)
+    case.cfg.metric_mode = 1
+    with gpu_block(case, uniform=True) as ref, gpu_block(case, uniform=True) as b:
+        shapes = b.autotune(reps=2)
+        assert all(s in (0, 1, 2) for s in shapes) and b.get_sweep_tiles() == shapes
+        ref.step(2)
+        b.step(2)
+        assert_same(b.download_f(), ref.download_f(), "f after autotune + 2 steps (tiles %s)" % shapes)
