@@ -194,6 +194,34 @@ int ocfd_show_flow_msg(ocfd_handle *h, double out[3]);
  * Enstrophy, d_rms, T_aver, Amu_aver, d_aver, Ux_rms, Alamdax. */
 int ocfd_ana_box(ocfd_handle *h, double out[20]);
 
+/* ---- batches of independent boxes ------------------------------------- */
+/* A batch advances many independent blocks ("boxes") of uneven size in the
+ * same time steps: each kernel phase of a step (halo fill, the three sweeps
+ * per scheme group, viscous fluxes, stage update, T <= 0 repair) is one
+ * launch over the tiles of all boxes, and whole steps replay from a captured
+ * hipGraph.  Boxes must be periodic on every axis with OCFD_METRIC_UNIFORM,
+ * without hybrid scheme, boundary hook, ghost cells or filter; each box keeps
+ * its own scheme, splitting, viscous order, Ma, Re, dt; all boxes share one
+ * precision.  Results equal stepping each box alone bit for bit.  No
+ * counterpart in the reference (one reference run advances one block). */
+typedef struct ocfd_batch ocfd_batch;
+int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocfd_batch **out);
+int ocfd_batch_destroy(ocfd_batch *b);
+/* The handle of box i (owned by the batch, do not destroy): set-up
+ * (ocfd_set_uniform_metrics, ocfd_upload_primitives, ocfd_bc), downloads and
+ * diagnostics of the box go through the single-block entry points. */
+int ocfd_batch_box(ocfd_batch *b, int32_t i, ocfd_handle **h);
+/* nsteps time steps of every box; returns the number of T <= 0 cells
+ * repaired (error if a box has more than 10 in one stage). */
+int ocfd_batch_step(ocfd_batch *b, int32_t nsteps);
+/* Autotuner: per scheme group and axis, the fastest sweep tile shape of the
+ * batched launch (overwrites the boxes' du). */
+int ocfd_batch_autotune(ocfd_batch *b, int32_t reps);
+int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape[3]);
+/* Number of scheme groups, their tile shapes (3 per group; shapes may be
+ * NULL) and the total cell count. */
+int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells);
+
 /* ---- instrumentation -------------------------------------------------- */
 /* Enable per-kernel HIP-event timing on the handle's stream. */
 int ocfd_profile_enable(ocfd_handle *h, int32_t on);

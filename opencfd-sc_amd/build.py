@@ -3,10 +3,11 @@
     python opencfd-sc_amd/build.py            # build (parallel per-file compile)
     python opencfd-sc_amd/build.py --clean
 
-FP64 kernels are compiled with -ffp-contract=off so that every floating-point
-expression rounds exactly as the reference's (no fused multiply-add); the FP32
-kernels of the mixed-precision form (CONTRACT) allow contraction.  Packed-FP32
-instructions are disabled everywhere.
+All kernels are compiled with -ffp-contract=off: the FP64 ones so that every
+floating-point expression rounds exactly as the reference's (no fused
+multiply-add), the FP32 ones of the mixed-precision form so that their
+rounding does not depend on the code around them (a box of a batch equals the
+same box stepped alone bit for bit).  Packed-FP32 instructions are disabled.
 """
 from __future__ import annotations
 
@@ -25,8 +26,7 @@ ARCH = "gfx950"
 
 SOURCES = ["ocfd_inviscid.hip", "ocfd_inviscid_mixed.hip", "ocfd_hybrid.hip", "ocfd_viscous.hip",
            "ocfd_viscous_mixed.hip", "ocfd_misc.hip", "ocfd_api.hip"]
-# FP32 kernels of the mixed-precision form: FMA contraction allowed
-CONTRACT = {"ocfd_inviscid_mixed.hip", "ocfd_viscous_mixed.hip"}
+
 FLAGS = [
     "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -37,8 +37,7 @@ FLAGS = [
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    flags = [("-ffp-contract=fast" if (f == "-ffp-contract=off" and src in CONTRACT) else f) for f in FLAGS]
-    cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <string>
@@ -823,6 +824,331 @@ int ocfd_profile_reset(ocfd_handle *h) {
   if (flush_prof(h)) return -1;
   h->acc.clear();
   return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches of independent periodic boxes (ocfd_batch_*).  Every box is an
+// ocfd_handle of its own (set-up, transfers and diagnostics go through the
+// single-block entry points); a batched time step launches each kernel phase
+// once over the tiles of all boxes, from a device table of the boxes' Block
+// descriptors, and is replayed from a captured hipGraph.
+// ===========================================================================
+struct ocfd_batch {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int prec = OCFD_PREC_FP64_CODE;
+  std::vector<ocfd_handle *> box;
+  Block *dblk = nullptr;      // device copy of the boxes' Block descriptors
+  int *dtab = nullptr;        // box lists and tile offsets of every launch
+  int *neg = nullptr;         // NEG_STRIDE ints per box
+  std::vector<int> tab;       // host image of dtab
+  struct Launch {             // one batched launch: boxes ids, offsets off
+    int ids = 0, off = 0, m = 0;
+    int64_t ntiles = 0;
+  };
+  struct SweepGroup {         // boxes sharing scheme and characteristic flag
+    int sch = 0;
+    bool ch = false;
+    std::vector<int> ids;
+    Launch l[3][NSWEEP_SHAPES];
+    int shape[3] = {0, 0, 0};
+  };
+  struct ViscGroup {          // boxes sharing IF_Viscous and the CD order
+    bool visc = false, cd8 = false;
+    std::vector<int> ids;
+    Launch l;
+  };
+  std::vector<SweepGroup> sweep;
+  std::vector<ViscGroup> vis;
+  Launch halo;
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  int64_t cells = 0;
+};
+
+namespace {
+
+BoxSet boxset(const ocfd_batch *b, const ocfd_batch::Launch &l) {
+  return BoxSet{b->dblk, b->dtab + l.ids, b->dtab + l.off, l.m};
+}
+
+// box list + tile offsets of one launch, appended to the host table
+ocfd_batch::Launch add_launch(ocfd_batch *b, const std::vector<int> &ids,
+                              const std::function<int64_t(const Geom &)> &tiles) {
+  ocfd_batch::Launch l;
+  l.m = (int)ids.size();
+  l.ids = (int)b->tab.size();
+  for (int i : ids) b->tab.push_back(i);
+  l.off = (int)b->tab.size();
+  int64_t t = 0;
+  for (int i : ids) {
+    b->tab.push_back((int)t);
+    t += tiles(b->box[i]->B.g);
+  }
+  b->tab.push_back((int)t);
+  l.ntiles = t;
+  return l;
+}
+
+void batch_invalidate(ocfd_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+}
+
+void batch_free(ocfd_batch *b) {
+  batch_invalidate(b);
+  for (ocfd_handle *h : b->box) ocfd_destroy(h);
+  if (b->dblk) (void)hipFree(b->dblk);
+  if (b->dtab) (void)hipFree(b->dtab);
+  if (b->neg) (void)hipFree(b->neg);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+int batch_sweep(ocfd_batch *b, const ocfd_batch::SweepGroup &g, int a, int shape) {
+  static const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
+  const ocfd_batch::Launch &l = g.l[a][shape];
+  const hipError_t e = launch_inviscid_batch(boxset(b, l), b->prec, a, g.sch, g.ch, modes[a], shape,
+                                             (int)l.ntiles, b->st);
+  return e == hipSuccess ? 0 : fail(std::string("batched sweep: ") + hipGetErrorString(e));
+}
+
+int batch_halo(ocfd_batch *b) {
+  const hipError_t e = launch_halo_batch(boxset(b, b->halo), (int)b->halo.ntiles, b->st);
+  return e == hipSuccess ? 0 : fail(std::string("batched halo fill: ") + hipGetErrorString(e));
+}
+
+// the three stages of one time step of every box (OCFD_NS_Solver.f90:33-60)
+int batch_step_body(ocfd_batch *b) {
+  for (int krk = 1; krk <= 3; krk++) {
+    if (batch_halo(b)) return -1;
+    for (const auto &g : b->sweep)
+      for (int a = 0; a < 3; a++)
+        if (batch_sweep(b, g, a, g.shape[a])) return -1;
+    for (const auto &v : b->vis) {
+      if (!v.visc) continue;
+      const hipError_t e = launch_visc_flux_batch(boxset(b, v.l), b->prec, v.cd8, (int)v.l.ntiles, b->st);
+      if (e != hipSuccess) return fail(std::string("batched viscous flux: ") + hipGetErrorString(e));
+    }
+    for (const auto &v : b->vis) {
+      const hipError_t e = launch_visc_div_update_batch(boxset(b, v.l), b->prec, v.visc, v.cd8, krk, b->neg,
+                                                        (int)v.l.ntiles, b->st);
+      if (e != hipSuccess) return fail(std::string("batched stage update: ") + hipGetErrorString(e));
+    }
+    const hipError_t e = launch_negt_repair_batch(b->dblk, (int)b->box.size(), b->neg, b->st);
+    if (e != hipSuccess) return fail(std::string("batched T <= 0 repair: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int batch_capture(ocfd_batch *b) {
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_step_body(b);
+  const hipError_t e = hipStreamEndCapture(b->st, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    b->gexec = nullptr;
+    return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  return 0;
+}
+
+// the boxes' descriptors and the launch tables to the device; the boxes'
+// own streams are drained first (set-up work may still run there)
+int batch_sync_tables(ocfd_batch *b) {
+  HIPCHK(hipSetDevice(b->device));
+  std::vector<Block> blk;
+  for (ocfd_handle *h : b->box) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    blk.push_back(h->B);
+  }
+  HIPCHK(hipMemcpy(b->dblk, blk.data(), blk.size() * sizeof(Block), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dtab, b->tab.data(), b->tab.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocfd_batch **out) {
+  if (!cfgs || !out || nbox < 1) return fail("ocfd_batch_create: need at least one box");
+  for (int i = 0; i < nbox; i++) {
+    const ocfd_config &c = cfgs[i];
+    const std::string at = "box " + std::to_string(i) + ": ";
+    if (c.iperiodic[0] != 1 || c.iperiodic[1] != 1 || c.iperiodic[2] != 1)
+      return fail(at + "batched boxes must be periodic on every axis");
+    if (c.metric_mode != OCFD_METRIC_UNIFORM) return fail(at + "batched boxes need OCFD_METRIC_UNIFORM");
+    if (c.scheme_invis == SCH_HYBRID) return fail(at + "the HYBRID scheme is not batched");
+    if (c.ibc != 0 || c.nfiltering != 0) return fail(at + "batched boxes take no boundary hook and no filter");
+    for (int q = 0; q < 6; q++)
+      if (c.ghost_cell[q] != 0) return fail(at + "batched boxes take no ghost cells");
+    if (c.precision != cfgs[0].precision) return fail(at + "all boxes of a batch share one precision");
+  }
+  ocfd_batch *b = new ocfd_batch();
+  b->device = device;
+  b->prec = cfgs[0].precision == OCFD_PREC_MIXED ? OCFD_PREC_MIXED_CODE : OCFD_PREC_FP64_CODE;
+  for (int i = 0; i < nbox; i++) {
+    ocfd_handle *h = nullptr;
+    if (ocfd_create(&cfgs[i], device, &h)) {
+      const std::string msg = "box " + std::to_string(i) + ": " + g_err;
+      batch_free(b);
+      return fail(msg);
+    }
+    b->box.push_back(h);
+    b->cells += (int64_t)cfgs[i].nx * cfgs[i].ny * cfgs[i].nz;
+  }
+  // launch groups
+  for (int i = 0; i < nbox; i++) {
+    const ocfd_config &c = cfgs[i];
+    const bool ch = c.if_scheme_character != 0;
+    auto it = std::find_if(b->sweep.begin(), b->sweep.end(),
+                           [&](const ocfd_batch::SweepGroup &g) { return g.sch == c.scheme_invis && g.ch == ch; });
+    if (it == b->sweep.end()) {
+      b->sweep.emplace_back();
+      b->sweep.back().sch = c.scheme_invis;
+      b->sweep.back().ch = ch;
+      it = b->sweep.end() - 1;
+    }
+    it->ids.push_back(i);
+    const bool visc = c.if_viscous == 1, cd8 = c.scheme_vis == SCH_CD8;
+    auto iv = std::find_if(b->vis.begin(), b->vis.end(),
+                           [&](const ocfd_batch::ViscGroup &g) { return g.visc == visc && g.cd8 == cd8; });
+    if (iv == b->vis.end()) {
+      b->vis.emplace_back();
+      b->vis.back().visc = visc;
+      b->vis.back().cd8 = cd8;
+      iv = b->vis.end() - 1;
+    }
+    iv->ids.push_back(i);
+  }
+  std::vector<int> all(nbox);
+  for (int i = 0; i < nbox; i++) all[i] = i;
+  b->halo = add_launch(b, all, [](const Geom &g) { return halo_tiles(g); });
+  for (auto &g : b->sweep)
+    for (int a = 0; a < 3; a++)
+      for (int sh = 0; sh < NSWEEP_SHAPES; sh++)
+        g.l[a][sh] = add_launch(b, g.ids, [a, sh](const Geom &gg) { return sweep_tiles(gg, a, sh); });
+  for (auto &v : b->vis) v.l = add_launch(b, v.ids, [](const Geom &g) { return march_tiles(g, batch_kseg(g)); });
+  int64_t maxt = 0;
+  for (int x : b->tab) maxt = std::max<int64_t>(maxt, x);
+  if (maxt >= INT32_MAX / 2) {
+    batch_free(b);
+    return fail("ocfd_batch_create: too many workgroups for one launch");
+  }
+  int rc = 0;
+  rc |= hipSetDevice(device) != hipSuccess;
+  rc |= hipStreamCreate(&b->st) != hipSuccess;
+  rc |= hipMalloc(&b->dblk, nbox * sizeof(Block)) != hipSuccess;
+  rc |= hipMalloc(&b->dtab, b->tab.size() * sizeof(int)) != hipSuccess;
+  rc |= hipMalloc(&b->neg, (size_t)NEG_STRIDE * nbox * sizeof(int)) != hipSuccess;
+  if (rc) {
+    batch_free(b);
+    return fail("ocfd_batch_create: device allocation failed");
+  }
+  b->use_graph = std::getenv("OCFD_NO_GRAPH") == nullptr;
+  *out = b;
+  return 0;
+}
+
+int ocfd_batch_destroy(ocfd_batch *b) {
+  if (b) batch_free(b);
+  return 0;
+}
+
+int ocfd_batch_box(ocfd_batch *b, int32_t i, ocfd_handle **h) {
+  if (i < 0 || i >= (int)b->box.size()) return fail("ocfd_batch_box: box index out of range");
+  *h = b->box[i];
+  return 0;
+}
+
+int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
+  const int nbox = (int)b->box.size();
+  if (batch_sync_tables(b)) return -1;
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((NEG_STRIDE * nbox + 127) / 128)), dim3(128), 0, b->st, b->neg,
+                     NEG_STRIDE * nbox);
+  for (int n = 0; n < nsteps; n++) {
+    if (b->use_graph) {
+      if (!b->gexec && batch_capture(b)) return -1;
+      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    } else if (batch_step_body(b)) {
+      return -1;
+    }
+    for (ocfd_handle *h : b->box) {
+      h->istep += 1;
+      h->tt = h->tt + h->cfg.dt;
+    }
+  }
+  std::vector<int> neg((size_t)NEG_STRIDE * nbox);
+  HIPCHK(hipMemcpyAsync(neg.data(), b->neg, neg.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  int total = 0;
+  for (int i = 0; i < nbox; i++) {
+    if (neg[(size_t)NEG_STRIDE * i + 66] > 10)
+      return fail("box " + std::to_string(i) + ": more than 10 cells with T <= 0 in one stage");
+    total += neg[(size_t)NEG_STRIDE * i + 65];
+  }
+  return total;
+}
+
+int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape[3]) {
+  if (group < 0 || group >= (int)b->sweep.size()) return fail("sweep group out of range");
+  for (int a = 0; a < 3; a++)
+    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0, 1 or 2");
+  for (int a = 0; a < 3; a++) b->sweep[group].shape[a] = shape[a];
+  batch_invalidate(b);
+  return 0;
+}
+
+int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells) {
+  if (ngroups) *ngroups = (int)b->sweep.size();
+  if (shapes)
+    for (size_t g = 0; g < b->sweep.size(); g++)
+      for (int a = 0; a < 3; a++) shapes[3 * g + a] = b->sweep[g].shape[a];
+  if (cells) *cells = b->cells;
+  return 0;
+}
+
+int ocfd_batch_autotune(ocfd_batch *b, int32_t reps) {
+  if (reps < 1) reps = 5;
+  if (batch_sync_tables(b)) return -1;
+  if (batch_halo(b)) return -1;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  for (auto &g : b->sweep) {
+    for (int a = 0; a < 3 && !rc; a++) {
+      float best = 0.f;
+      int bests = 0;
+      for (int sh = 0; sh < NSWEEP_SHAPES && !rc; sh++) {
+        rc = batch_sweep(b, g, a, sh);  // warm-up
+        if (!rc && hipEventRecord(e0, b->st) != hipSuccess) rc = fail("hipEventRecord failed");
+        for (int r = 0; r < reps && !rc; r++) rc = batch_sweep(b, g, a, sh);
+        float ms = 0.f;
+        if (!rc && (hipEventRecord(e1, b->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = fail("autotune: event timing failed");
+        if (!rc && (sh == 0 || ms < best)) {
+          best = ms;
+          bests = sh;
+        }
+      }
+      g.shape[a] = bests;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  batch_invalidate(b);
+  return rc ? -1 : 0;
 }
 
 }  // extern "C"

@@ -14,4 +14,14 @@ hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, h
   return launch_sweep<double>(B, axis, mode, tile.shape, st);
 }
 
+hipError_t launch_inviscid_batch_mixed(const BoxSet &S, int axis, int sch, bool ch, int mode, int shape, int ntiles,
+                                       hipStream_t st);
+
+hipError_t launch_inviscid_batch(const BoxSet &S, int prec, int axis, int sch, bool ch, int mode, int shape,
+                                 int ntiles, hipStream_t st) {
+  if (shape < 0 || shape >= NSWEEP_SHAPES) return hipErrorInvalidValue;
+  if (prec == OCFD_PREC_MIXED_CODE) return launch_inviscid_batch_mixed(S, axis, sch, ch, mode, shape, ntiles, st);
+  return launch_sweep_batch<double>(S, axis, sch, ch, mode, shape, ntiles, st);
+}
+
 }  // namespace ocfd

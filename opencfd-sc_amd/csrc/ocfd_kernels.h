@@ -36,6 +36,30 @@ __device__ inline int xcd_tile(int b, int nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
+// ints per block of the T <= 0 bookkeeping: [0] cells found in the current
+// stage, [1..64] their interior indices, [65] total, [66] largest per stage
+constexpr int NEG_STRIDE = 72;
+
+// The boxes of one batched launch (ocfd_batch): workgroup tiles of box
+// ids[j] are off[j] .. off[j+1]-1 of the launch.  box() finds the box of a
+// tile by binary search over the m entries (the tile index is uniform over
+// the workgroup, so the search runs on scalar registers).
+struct BoxSet {
+  const Block *blk;  // the batch's blocks (device table)
+  const int *ids;    // boxes of this launch
+  const int *off;    // m + 1 tile offsets
+  int m;
+  __device__ inline int entry(int t) const {
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+};
+
 // Index arithmetic of a sweep along AXIS: point s of line w, third index c
 // (x: lines j, third k; y: lines i, third k; z: lines i, third j).
 template <int AXIS>
@@ -87,10 +111,37 @@ constexpr int NSWEEP_SHAPES = 3;
 struct SweepTile {
   int shape;
 };
+// points along the axis of a tile of `shape` (256 threads: lines = 256 / TL)
+inline int sweep_shape_tl(int axis, int shape) {
+  return axis == 0 ? (shape == 1 ? 32 : (shape == 2 ? 128 : 64)) : (shape == 1 ? 16 : (shape == 2 ? 64 : 32));
+}
+// workgroups of one sweep of a block
+inline int64_t sweep_tiles(const Geom &g, int axis, int shape) {
+  const int tl = sweep_shape_tl(axis, shape), tw = 256 / tl;
+  const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  const int nw = axis == 0 ? g.ny : g.nx;
+  const int nc = axis == 2 ? g.ny : g.nz;
+  return (int64_t)((n + tl - 1) / tl) * ((nw + tw - 1) / tw) * nc;
+}
 
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, hipStream_t st);
 hipError_t launch_hybrid_lists(const Block &B, hipStream_t st);
+// batched launches over the boxes of S (ntiles workgroups in all)
+hipError_t launch_inviscid_batch(const BoxSet &S, int prec, int axis, int sch, bool ch, int mode, int shape,
+                                 int ntiles, hipStream_t st);
+hipError_t launch_visc_flux_batch(const BoxSet &S, int prec, bool cd8, int ntiles, hipStream_t st);
+hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bool cd8, int krk, int *neg,
+                                        int ntiles, hipStream_t st);
+hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st);
+hipError_t launch_negt_repair_batch(const Block *blk, int nbox, int *neg, hipStream_t st);
+// planes per workgroup of the k-marching kernels for a box of a batch, and
+// their workgroup count (the batch fills the chip with many boxes, so each
+// box is marched in long segments)
+int batch_kseg(const Geom &g);
+int64_t march_tiles(const Geom &g, int kseg);
+// workgroups of the periodic halo fill of one box
+int64_t halo_tiles(const Geom &g);
 hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st);
 hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st);
 hipError_t launch_sensor(const Block &B, hipStream_t st);

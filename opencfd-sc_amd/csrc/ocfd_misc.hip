@@ -31,12 +31,12 @@ __device__ inline void cell_of(const Geom &g, int64_t t, int &i, int &j, int &k)
 struct HaloFields {
   double *p[8];
 };
-__global__ void k_halo(const Geom g, const HaloFields F, const int nf, const int px, const int py, const int pz) {
+__device__ inline void halo_point(const Geom &g, double *const *P, const int nf, const int px, const int py,
+                                  const int pz, const int64_t t) {
   const int64_t nxh = px ? (int64_t)2 * LAP * g.ny * g.nz : 0;
   const int64_t nyh = py ? (int64_t)2 * LAP * g.nx * g.nz : 0;
   const int64_t nzh = pz ? (int64_t)2 * LAP * g.nx * g.ny : 0;
   const int64_t tot = nxh + nyh + nzh;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tot) return;
   int64_t dst, src;
   if (t < nxh) {
@@ -60,7 +60,19 @@ __global__ void k_halo(const Geom g, const HaloFields F, const int nf, const int
     if (l < LAP) { dst = g.hi(i, j, g.nz + 1 + l); src = g.hi(i, j, 1 + l); }
     else { const int q = l - LAP; dst = g.hi(i, j, 1 - LAP + q); src = g.hi(i, j, g.nz - LAP + 1 + q); }
   }
-  for (int q = 0; q < nf; q++) F.p[q][dst] = F.p[q][src];
+  for (int q = 0; q < nf; q++) P[q][dst] = P[q][src];
+}
+
+__global__ void k_halo(const Geom g, const HaloFields F, const int nf, const int px, const int py, const int pz) {
+  halo_point(g, F.p, nf, px, py, pz, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the five primitives of every box of a batch (all axes periodic)
+__global__ __launch_bounds__(NT) void k_halo_batch(const BoxSet S) {
+  const int j = S.entry(blockIdx.x);
+  const Block &B = S.blk[S.ids[j]];
+  double *const P[5] = {B.d, B.u, B.v, B.w, B.T};
+  halo_point(B.g, P, 5, 1, 1, 1, (int64_t)(blockIdx.x - S.off[j]) * NT + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,8 +248,7 @@ __global__ __launch_bounds__(NT) void k_duvwt(const Block B, int *neg) {
 // T <= 0 repair (OCFD_handle_NegativeT.f90:11): T = mean of the six
 // neighbours; the listed cells are processed in ascending index order by one
 // thread.  Neighbour values are those after the whole primitive update.
-__global__ void k_negt_repair(const Block B, int *neg, const int refresh) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void negt_repair_body(const Block &B, int *neg, const int refresh) {
   int cnt = neg[0];
   if (cnt <= 0) return;
   if (cnt > 64) cnt = 64;
@@ -264,6 +275,17 @@ __global__ void k_negt_repair(const Block B, int *neg, const int refresh) {
   neg[65] += total;
   if (total > neg[66]) neg[66] = total;
   neg[0] = 0;
+}
+
+__global__ void k_negt_repair(const Block B, int *neg, const int refresh) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  negt_repair_body(B, neg, refresh);
+}
+
+// one workgroup per box of a batch
+__global__ void k_negt_repair_batch(const Block *blk, int *neg) {
+  if (threadIdx.x != 0) return;
+  negt_repair_body(blk[blockIdx.x], neg + NEG_STRIDE * blockIdx.x, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,6 +740,20 @@ hipError_t launch_init_f(const Block &B, hipStream_t st) {
 
 hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st) {
   hipLaunchKernelGGL(k_duvwt, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, neg);
+  return hipGetLastError();
+}
+
+int64_t halo_tiles(const Geom &g) {
+  return nb(2 * LAP * ((int64_t)g.ny * g.nz + (int64_t)g.nx * g.nz + (int64_t)g.nx * g.ny));
+}
+
+hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_halo_batch, dim3((unsigned)ntiles), dim3(NT), 0, st, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_negt_repair_batch(const Block *blk, int nbox, int *neg, hipStream_t st) {
+  hipLaunchKernelGGL(k_negt_repair_batch, dim3((unsigned)nbox), dim3(64), 0, st, blk, neg);
   return hipGetLastError();
 }
 

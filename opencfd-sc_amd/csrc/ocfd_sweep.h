@@ -35,10 +35,11 @@ __device__ constexpr inline int lidx(int m, int p, int wl) {
   return AXIS == 0 ? (m * TW + wl) * NPT + p : (m * NPT + p) * TW + wl;
 }
 
-// FAST: the sweep axis is periodic (no block-edge closures) and the metrics
-// are uniform, both known at compile time.
+// One tile b of a sweep of block B.  FAST: the sweep axis is periodic (no
+// block-edge closures) and the metrics are uniform, both known at compile
+// time.
 template <int AXIS, int SCH, bool CHAR, int TL, int TW, bool FAST, typename R>
-__global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int mode) {
+__device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode) {
   typedef TileIndex<AXIS> TI;
   constexpr int NP = TL + 8;  // points s0-4 .. s0+TL+3
   constexpr int NF = TL + 1;  // faces  s0-1 .. s0+TL-1
@@ -56,7 +57,6 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   const int n = g.n(AXIS);
   const int nw = TI::nwidth(g), nc = TI::nthird(g);
   const int nts = (n + TL - 1) / TL, ntw = (nw + TW - 1) / TW;
-  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int ts = b % nts; b /= nts;
   const int tw = b % ntw;
   const int c = b / ntw + 1;
@@ -357,6 +357,21 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int 
   }
 }
 
+// one block: the tiles in XCD order
+template <int AXIS, int SCH, bool CHAR, int TL, int TW, bool FAST, typename R>
+__global__ __launch_bounds__(NTHREADS) void k_inviscid(const Block B, const int mode) {
+  sweep_tile<AXIS, SCH, CHAR, TL, TW, FAST, R>(B, xcd_tile(blockIdx.x, gridDim.x), mode);
+}
+
+// a batch of periodic boxes with uniform metrics: the tiles of all boxes of
+// the launch, in XCD order
+template <int AXIS, int SCH, bool CHAR, int TL, int TW, typename R>
+__global__ __launch_bounds__(NTHREADS) void k_inviscid_batch(const BoxSet S, const int mode) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = S.entry(t);
+  sweep_tile<AXIS, SCH, CHAR, TL, TW, true, R>(S.blk[S.ids[j]], t - S.off[j], mode);
+}
+
 template <int TL, int TW, bool CHAR, bool HYB, typename R>
 constexpr size_t sweep_lds(int nq) {
   return sizeof(R) * (10 * (TL + 8) * TW + (CHAR && !HYB ? nq * (TL + 8) * TW : 0) + (CHAR ? 5 : 10) * (TL + 1) * TW) +
@@ -426,6 +441,41 @@ hipError_t launch_sweep(const Block &B, int axis, int mode, int shape, hipStream
   if (axis == 0) return dispatch_axis<0, R>(B, mode, shape, st);
   if (axis == 1) return dispatch_axis<1, R>(B, mode, shape, st);
   return dispatch_axis<2, R>(B, mode, shape, st);
+}
+
+// ---- batches (ocfd_batch): periodic boxes with uniform metrics, no hybrid --
+template <int AXIS, typename R, int SHAPE>
+hipError_t batch_axis(const BoxSet &S, int sch, bool ch, int mode, int ntiles, hipStream_t st) {
+  constexpr int TL = shape_tl<AXIS>(SHAPE);
+  constexpr int TW = NTHREADS / TL;
+  const size_t lds = ch ? sweep_lds<TL, TW, true, false, R>(4) : sweep_lds<TL, TW, false, false, R>(4);
+#define OCFD_BATCH_SWEEP(SCH_, CH_)                                                                         \
+  hipLaunchKernelGGL((k_inviscid_batch<AXIS, SCH_, CH_, TL, TW, R>), dim3((unsigned)ntiles), dim3(NTHREADS), lds, st, \
+                     S, mode)
+  switch (sch) {
+    case SCH_WENO5: if (ch) OCFD_BATCH_SWEEP(SCH_WENO5, true); else OCFD_BATCH_SWEEP(SCH_WENO5, false); break;
+    case SCH_WENO7: if (ch) OCFD_BATCH_SWEEP(SCH_WENO7, true); else OCFD_BATCH_SWEEP(SCH_WENO7, false); break;
+    case SCH_OMP6: if (ch) OCFD_BATCH_SWEEP(SCH_OMP6, true); else OCFD_BATCH_SWEEP(SCH_OMP6, false); break;
+    case SCH_UD7L: if (ch) OCFD_BATCH_SWEEP(SCH_UD7L, true); else OCFD_BATCH_SWEEP(SCH_UD7L, false); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef OCFD_BATCH_SWEEP
+  return hipGetLastError();
+}
+
+template <int AXIS, typename R>
+hipError_t batch_shape(const BoxSet &S, int sch, bool ch, int mode, int shape, int ntiles, hipStream_t st) {
+  if (shape == 1) return batch_axis<AXIS, R, 1>(S, sch, ch, mode, ntiles, st);
+  if (shape == 2) return batch_axis<AXIS, R, 2>(S, sch, ch, mode, ntiles, st);
+  return batch_axis<AXIS, R, 0>(S, sch, ch, mode, ntiles, st);
+}
+
+template <typename R>
+hipError_t launch_sweep_batch(const BoxSet &S, int axis, int sch, bool ch, int mode, int shape, int ntiles,
+                              hipStream_t st) {
+  if (axis == 0) return batch_shape<0, R>(S, sch, ch, mode, shape, ntiles, st);
+  if (axis == 1) return batch_shape<1, R>(S, sch, ch, mode, shape, ntiles, st);
+  return batch_shape<2, R>(S, sch, ch, mode, shape, ntiles, st);
 }
 
 }  // namespace

@@ -75,10 +75,9 @@ struct TileCols {
   int i0, j0, tx, ty, i, j, kb, ke;
   bool valid;
 };
-__device__ inline TileCols tile_cols(const Geom &g, int kseg) {
+__device__ inline TileCols tile_cols(const Geom &g, int kseg, int b) {
   TileCols t;
   const int ntx = (g.nx + TXT - 1) / TXT, nty = (g.ny + TYT - 1) / TYT;
-  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % ntx;
   b /= ntx;
   const int by = b % nty;
@@ -110,13 +109,13 @@ __device__ inline int wrap_index(int t, int n, int R) {
 // basic block, and the loads of plane k+1 issued at its top are waited for
 // only after the arithmetic of plane k.
 template <bool CD8, bool FAST, typename RT>
-__global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg) {
+__device__ __forceinline__ void visc_flux_tile(const Block &B, const int kseg, const int tile) {
   typedef PlaneTile<CD8> PT;
   constexpr int R = PT::R, LX = PT::LX, LY = PT::LY, NPL = LX * LY, Q = 2 * R + 1;
   constexpr int NRF = (NPL + NT - 1) / NT;
   __shared__ RT sT[2][4 * NPL];  // u, v, w, T of one plane: [4][LY][LX]
   const Geom &g = B.g;
-  const TileCols tc = tile_cols(g, kseg);
+  const TileCols tc = tile_cols(g, kseg, tile);
   const double *F[4] = {B.u, B.v, B.w, B.T};
   const int64_t sk = g.hstride(2);
   const int ic = tc.i < g.nx ? tc.i : g.nx, jc = tc.j < g.ny ? tc.j : g.ny;
@@ -253,6 +252,11 @@ __global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg)
   }
 }
 
+template <bool CD8, bool FAST, typename RT>
+__global__ __launch_bounds__(NT) void k_visc_flux(const Block B, const int kseg) {
+  visc_flux_tile<CD8, FAST, RT>(B, kseg, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 // viscous divergence of component m (1..4) at cell t (OCFD_viscous.f90:114-129)
 template <typename RT>
 __device__ inline RT visc_div_m(const Block &B, int m, int i, int j, int k) {
@@ -310,8 +314,8 @@ __device__ inline void refresh_of(double cv, double d, double u, double v, doubl
 // du, f, fn of the own cell and the Ev tiles -- are all issued at the top of
 // iteration k to keep enough bytes in flight.
 template <bool VISC, bool CD8, bool FAST, typename RT>
-__global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int krk, const int refresh,
-                                                        const int kseg, int *neg) {
+__device__ __forceinline__ void visc_div_update_tile(const Block &B, const int krk, const int refresh, const int kseg,
+                                                     int *neg, const int tile) {
   typedef PlaneTile<CD8> PT;
   constexpr int R = PT::R, LX = PT::LX, LY = PT::LY, Q = 2 * R + 1;
   constexpr int N1 = LX * TYT, N2 = TXT * LY;       // points per component
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
   const RT *const ev = reinterpret_cast<const RT *>(B.ev);
   const RT *const dus = reinterpret_cast<const RT *>(B.du);  // du in the right-hand side's real type
   const Geom &g = B.g;
-  const TileCols tc = tile_cols(g, kseg);
+  const TileCols tc = tile_cols(g, kseg, tile);
   const int nz = g.nz;
   const int64_t ni = g.ni, skz = g.istride(2);
   const bool perx = FAST || B.S.iperiodic[0] == 1, pery = FAST || B.S.iperiodic[1] == 1,
@@ -503,6 +507,34 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
   }
 }
 
+template <bool VISC, bool CD8, bool FAST, typename RT>
+__global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int krk, const int refresh,
+                                                        const int kseg, int *neg) {
+  visc_div_update_tile<VISC, CD8, FAST, RT>(B, krk, refresh, kseg, neg, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batches of periodic boxes with uniform metrics (ocfd_batch): the tiles of
+// all boxes of the launch in XCD order; box b marches segments of
+// batch_kseg(g) planes and counts its T <= 0 cells in neg + NEG_STRIDE * b.
+__device__ inline int batch_kseg_dev(const Geom &g) { return g.nz < KSEG_MAX ? g.nz : KSEG_MAX; }
+
+template <bool CD8, typename RT>
+__global__ __launch_bounds__(NT) void k_visc_flux_batch(const BoxSet S) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = S.entry(t);
+  const Block &B = S.blk[S.ids[j]];
+  visc_flux_tile<CD8, true, RT>(B, batch_kseg_dev(B.g), t - S.off[j]);
+}
+
+template <bool VISC, bool CD8, typename RT>
+__global__ __launch_bounds__(NT) void k_visc_div_update_batch(const BoxSet S, const int krk, int *neg) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = S.entry(t);
+  const int box = S.ids[j];
+  const Block &B = S.blk[box];
+  visc_div_update_tile<VISC, CD8, true, RT>(B, krk, 1, batch_kseg_dev(B.g), neg + NEG_STRIDE * box, t - S.off[j]);
+}
+
 // ---------------------------------------------------------------------------
 // launch shapes and per-precision launchers (RT = double, or float for the
 // mixed-precision form; each precision is instantiated in its own
@@ -557,6 +589,25 @@ hipError_t visc_div_update_as(const Block &B, int krk, int refresh_planes, int *
   else if (per) OCFD_DIVUPD(true, false, true);
   else OCFD_DIVUPD(true, false, false);
 #undef OCFD_DIVUPD
+  return hipGetLastError();
+}
+
+template <typename RT>
+hipError_t visc_flux_batch_as(const BoxSet &S, bool cd8, int ntiles, hipStream_t st) {
+  if (cd8) hipLaunchKernelGGL((k_visc_flux_batch<true, RT>), dim3((unsigned)ntiles), dim3(NT), 0, st, S);
+  else hipLaunchKernelGGL((k_visc_flux_batch<false, RT>), dim3((unsigned)ntiles), dim3(NT), 0, st, S);
+  return hipGetLastError();
+}
+
+template <typename RT>
+hipError_t visc_div_update_batch_as(const BoxSet &S, bool visc, bool cd8, int krk, int *neg, int ntiles,
+                                    hipStream_t st) {
+#define OCFD_DIVUPD_B(V, C) \
+  hipLaunchKernelGGL((k_visc_div_update_batch<V, C, RT>), dim3((unsigned)ntiles), dim3(NT), 0, st, S, krk, neg)
+  if (!visc) OCFD_DIVUPD_B(false, false);
+  else if (cd8) OCFD_DIVUPD_B(true, true);
+  else OCFD_DIVUPD_B(true, false);
+#undef OCFD_DIVUPD_B
   return hipGetLastError();
 }
 

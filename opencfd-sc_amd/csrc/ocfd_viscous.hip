@@ -1,8 +1,7 @@
 // ocfd_viscous.hip -- launchers of the viscous right-hand side (du_viscous,
 // OCFD_viscous.f90:29-149) and of the fused stage update; the FP64
 // instantiations of the kernels in ocfd_visc.h.  The mixed-precision form's
-// FP32 instantiations live in ocfd_viscous_mixed.hip, built with FMA
-// contraction allowed.
+// FP32 instantiations live in ocfd_viscous_mixed.hip.
 #include "ocfd_visc.h"
 
 namespace ocfd {
@@ -10,6 +9,9 @@ namespace ocfd {
 hipError_t launch_visc_flux_mixed(const Block &B, hipStream_t st);
 hipError_t launch_visc_div_mixed(const Block &B, hipStream_t st);
 hipError_t launch_visc_div_update_mixed(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st);
+hipError_t launch_visc_flux_batch_mixed(const BoxSet &S, bool cd8, int ntiles, hipStream_t st);
+hipError_t launch_visc_div_update_batch_mixed(const BoxSet &S, bool visc, bool cd8, int krk, int *neg, int ntiles,
+                                              hipStream_t st);
 
 hipError_t launch_visc_flux(const Block &B, hipStream_t st) {
   if (B.prec == OCFD_PREC_MIXED_CODE) return launch_visc_flux_mixed(B, st);
@@ -25,5 +27,19 @@ hipError_t launch_visc_div_update(const Block &B, int krk, int refresh_planes, i
   if (B.prec == OCFD_PREC_MIXED_CODE) return launch_visc_div_update_mixed(B, krk, refresh_planes, neg, st);
   return visc_div_update_as<double>(B, krk, refresh_planes, neg, st);
 }
+
+hipError_t launch_visc_flux_batch(const BoxSet &S, int prec, bool cd8, int ntiles, hipStream_t st) {
+  if (prec == OCFD_PREC_MIXED_CODE) return launch_visc_flux_batch_mixed(S, cd8, ntiles, st);
+  return visc_flux_batch_as<double>(S, cd8, ntiles, st);
+}
+
+hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bool cd8, int krk, int *neg,
+                                        int ntiles, hipStream_t st) {
+  if (prec == OCFD_PREC_MIXED_CODE) return launch_visc_div_update_batch_mixed(S, visc, cd8, krk, neg, ntiles, st);
+  return visc_div_update_batch_as<double>(S, visc, cd8, krk, neg, ntiles, st);
+}
+
+int batch_kseg(const Geom &g) { return g.nz < KSEG_MAX ? g.nz : KSEG_MAX; }
+int64_t march_tiles(const Geom &g, int kseg) { return march_blocks(g, kseg); }
 
 }  // namespace ocfd

@@ -1,8 +1,6 @@
 // ocfd_viscous_mixed.hip -- FP32 instantiations of the viscous kernels for
 // the mixed-precision form (Ev, the derivatives, stresses and fluxes in FP32;
-// du accumulated in FP32; the stage combination and primitives FP64).  Built
-// with FMA contraction allowed: this form is not bit-matched to the
-// reference.
+// du accumulated in FP32; the stage combination and primitives FP64).
 #include "ocfd_visc.h"
 
 namespace ocfd {
@@ -13,6 +11,15 @@ hipError_t launch_visc_div_mixed(const Block &B, hipStream_t st) { return visc_d
 
 hipError_t launch_visc_div_update_mixed(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
   return visc_div_update_as<float>(B, krk, refresh_planes, neg, st);
+}
+
+hipError_t launch_visc_flux_batch_mixed(const BoxSet &S, bool cd8, int ntiles, hipStream_t st) {
+  return visc_flux_batch_as<float>(S, cd8, ntiles, st);
+}
+
+hipError_t launch_visc_div_update_batch_mixed(const BoxSet &S, bool visc, bool cd8, int krk, int *neg, int ntiles,
+                                              hipStream_t st) {
+  return visc_div_update_batch_as<float>(S, visc, cd8, krk, neg, ntiles, st);
 }
 
 }  // namespace ocfd

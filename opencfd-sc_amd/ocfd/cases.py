@@ -219,11 +219,12 @@ def tgv(n=64, ma=0.1, re=1600.0, pr=0.7, gamma=1.4, cfl=0.5, scheme="WENO5", spl
 
 def isotropic(n=256, seed=20261202, k0=8.0, mt=0.5, re_lambda=72.0, cfl=0.5,
               scheme="HYBRID", split=2, char=1, hybrid_thresholds=(0.02, 0.1),
-              metric_mode=0) -> Case:
+              metric_mode=0, re=None) -> Case:
     """Decaying compressible isotropic turbulence: random solenoidal velocity
     with E(k) ~ k^4 exp(-2k^2/k0^2), uniform rho = T = 1, scaled so that
     <|u|^2> = 1 and M_t = sqrt(<|u|^2>)/<c> = mt (so Ma = mt), and Re chosen so
-    that Re_lambda (ana_box definition, OCFD_ANA.f90:265-272) equals re_lambda.
+    that Re_lambda (ana_box definition, OCFD_ANA.f90:265-272) equals re_lambda
+    (or Re = re when given).
     Grid: uniform box (Iflag_Gridtype = 0)."""
     X, Y, Z, _ = box_coords(n, n, n)
     u, v, w = solenoidal_field(n, seed, k0)
@@ -239,7 +240,8 @@ def isotropic(n=256, seed=20261202, k0=8.0, mt=0.5, re_lambda=72.0, cfl=0.5,
     del uh
     ux_rms = np.sqrt(np.mean(ux * ux))
     vv2 = np.mean(u * u + v * v + w * w) / 3.0
-    re = re_lambda * ux_rms / vv2
+    if re is None:
+        re = re_lambda * ux_rms / vv2
     d = np.ones_like(X)
     T = np.ones_like(X)
     cfg = CaseConfig(nx=n, ny=n, nz=n, iperiodic=(1, 1, 1), scheme_invis=scheme,

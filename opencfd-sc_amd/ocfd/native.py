@@ -301,6 +301,95 @@ class Block(BlockBase):
         self.close()
 
 
+class BoxView(Block):
+    """A box of a :class:`Batch`: the single-block interface on a handle the
+    batch owns (it is not destroyed here)."""
+
+    def __init__(self, lib, cfg: CaseConfig, h):
+        BlockBase.__init__(self, lib, cfg)
+        self.h = h
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """ocfd_batch_*: independent periodic boxes advanced together, one launch
+    per kernel phase over all boxes."""
+
+    def __init__(self, cfgs, device: int = 0, lib=None):
+        self.lib = lib or load_library()
+        self.cfgs = list(cfgs)
+        n = len(self.cfgs)
+        arr = (OcfdConfigC * n)(*[c.to_c() for c in self.cfgs])
+        h = ctypes.c_void_p()
+        rc = self.lib.ocfd_batch_create(arr, ctypes.c_int32(n), ctypes.c_int32(device), ctypes.byref(h))
+        if rc != 0:
+            raise OcfdError("ocfd_batch_create failed (%d): %s" % (rc, self._err()))
+        self.h = h
+        self.boxes = []
+        for i in range(n):
+            hb = ctypes.c_void_p()
+            self._call("box", ctypes.c_int32(i), ctypes.byref(hb))
+            self.boxes.append(BoxView(self.lib, self.cfgs[i], hb))
+
+    def _err(self):
+        msg = self.lib.ocfd_last_error()
+        return msg.decode() if msg else ""
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, "ocfd_batch_" + name)(self.h, *args)
+        if rc < 0:
+            raise OcfdError("ocfd_batch_%s failed (%d): %s" % (name, rc, self._err()))
+        return rc
+
+    def box(self, i) -> BoxView:
+        return self.boxes[i]
+
+    def step(self, nsteps=1):
+        return self._call("step", ctypes.c_int32(nsteps))
+
+    def autotune(self, reps=5):
+        self._call("autotune", ctypes.c_int32(reps))
+        return self.info()[1]
+
+    def set_sweep_tiles(self, group, shapes):
+        a = (ctypes.c_int32 * 3)(*[int(x) for x in shapes])
+        return self._call("set_sweep_tiles", ctypes.c_int32(group), a)
+
+    def info(self):
+        """(number of scheme groups, their tile shapes, total cells)"""
+        ng = ctypes.c_int32()
+        cells = ctypes.c_int64()
+        self._call("info", ctypes.byref(ng), None, ctypes.byref(cells))
+        sh = (ctypes.c_int32 * (3 * max(ng.value, 1)))()
+        self._call("info", ctypes.byref(ng), sh, ctypes.byref(cells))
+        return ng.value, [list(sh[3 * g:3 * g + 3]) for g in range(ng.value)], cells.value
+
+    def synchronize(self):
+        for b in self.boxes:
+            b.synchronize()
+
+    def close(self):
+        if self.h is not None:
+            for b in self.boxes:
+                b.close()
+            self.lib.ocfd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def setup_block(blk: BlockBase, case, use_uniform=None):
     """Grid + initial state as init3d + the first OCFD_bc (OCFD_NS_Solver.f90:14-15)."""
     cfg = case.cfg

@@ -1,0 +1,88 @@
+"""Batched boxes (ocfd_batch_*) against the same boxes stepped one at a time.
+
+A batch launches every kernel phase once over the tiles of all its boxes;
+per cell the arithmetic is the single-block kernels', so each box of a batch
+must equal the same box advanced alone through ocfd_step bit for bit, in FP64
+and in the mixed-precision form, with the default and the autotuned tiles.
+The single-block path is itself pinned to the reference (test_gpu_parity).
+"""
+import numpy as np
+import pytest
+
+from ocfd import cases, native
+from ocfd.config import PREC_FP64, PREC_MIXED
+
+pytestmark = pytest.mark.gpu
+
+# uneven periodic boxes: sizes not multiples of the tiles, schemes, splittings,
+# viscous orders, one inviscid box
+BOXES = [
+    dict(shape=(16, 16, 16), seed=81, scheme="WENO5", split=1, char=0),
+    dict(shape=(20, 28, 24), seed=82, scheme="OMP6", split=1, char=0, noise=0.03),
+    dict(shape=(36, 20, 40), seed=83, scheme="WENO7", split=2, char=1),
+    dict(shape=(24, 24, 24), seed=84, scheme="UD7L", split=2, char=0, vis="CD8"),
+    dict(shape=(18, 40, 22), seed=85, scheme="WENO5", split=1, char=1, ma=0.5),
+    dict(shape=(32, 24, 48), seed=86, scheme="WENO5", split=1, char=0, inviscid=True),
+]
+
+
+def make_cases(precision):
+    out = []
+    for kw in BOXES:
+        kw = dict(kw)
+        inviscid = kw.pop("inviscid", False)
+        case = cases.random_case(**kw)
+        case.uniform_metrics = cases.uniform_box_metrics(case.cfg.nx, case.cfg.ny, case.cfg.nz)
+        case.cfg = case.cfg.replace(metric_mode=1, precision=precision, if_viscous=0 if inviscid else 1)
+        out.append(case)
+    return out
+
+
+def single_runs(cs, nsteps):
+    res = []
+    for c in cs:
+        with native.setup_block(native.Block(c.cfg), c, use_uniform=True) as b:
+            b.step(nsteps)
+            res.append((b.download_f(), b.download_primitives()))
+    return res
+
+
+def check_equal(batch, ref, what):
+    for i, (f, prims) in enumerate(ref):
+        bx = batch.box(i)
+        got = bx.download_f()
+        nbad = int(np.count_nonzero(got != f))
+        assert nbad == 0, "%s: box %d f differs in %d values" % (what, i, nbad)
+        gp = bx.download_primitives()
+        for q in range(5):
+            assert np.array_equal(gp[q], prims[q]), "%s: box %d primitive %d" % (what, i, q)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_equals_single_boxes(precision):
+    cs = make_cases(precision)
+    ref3 = single_runs(cs, 3)
+    ref5 = single_runs(cs, 5)
+    with native.Batch([c.cfg for c in cs]) as batch:
+        for i, c in enumerate(cs):
+            native.setup_block(batch.box(i), c, use_uniform=True)
+        ngroups, shapes, cells = batch.info()
+        assert ngroups == 5 and cells == sum(c.cfg.nx * c.cfg.ny * c.cfg.nz for c in cs)
+        assert batch.step(3) == 0
+        check_equal(batch, ref3, "3 batched steps")
+        # the autotuner changes only launch shapes
+        batch.autotune(reps=2)
+        batch.step(2)
+        check_equal(batch, ref5, "2 more steps after autotune")
+        assert batch.box(0).get_time()[0] == 5
+
+
+def test_batch_rejects_unsupported_boxes():
+    c = cases.random_case(shape=(16, 16, 16), seed=87, scheme="HYBRID", split=2, char=1)
+    c.cfg = c.cfg.replace(metric_mode=1)
+    with pytest.raises(native.OcfdError, match="HYBRID"):
+        native.Batch([c.cfg])
+    c2 = cases.random_case(shape=(16, 16, 16), seed=88, iperiodic=(1, 0, 1))
+    c2.cfg = c2.cfg.replace(metric_mode=1)
+    with pytest.raises(native.OcfdError, match="periodic"):
+        native.Batch([c2.cfg])

@@ -14,8 +14,9 @@ Tolerances, relative L2 per field:
     scheme, whose face scheme is a discontinuous function of the state: a
     face whose sensor value lies within the mixed form's perturbation of a
     threshold switches between UD7L and WENO7, changing its flux by the
-    difference of the two reconstructions (on this strongly shocked 64^3
-    field a few such switches give ~2e-5; the bench reports the 512^3 case);
+    difference of the two reconstructions (a few such switches on this
+    strongly shocked 64^3 field raise the difference by an order of
+    magnitude; the bench reports the 512^3 case);
     kinetic energy and enstrophy <= 1e-6 relative in both;
   one right-hand side du on random blocks with block-edge closures and
     curvilinear metrics: <= 2e-4 -- du is a difference of neighbouring face
