@@ -193,7 +193,7 @@ def run_batch(args):
                    "scheme": "WENO5 or OMP6 per box, Steger-Warming, CD6, RK3 CFL 0.5",
                    "precision": "FP64" if prec == PREC_FP64 else "mixed",
                    "execution": "one launch per kernel phase over all boxes, graph-captured steps",
-                   "sweep_tiles_per_group": tiles},
+                   "autotuned": tiles},
         "one_at_a_time": {"value": round(single, 3), "ms_per_step": round(el_s / args.steps * 1e3, 4),
                           "batched_speedup": round(el_s / el_b, 4)},
         "roofline": None, "cpu_baseline": None,

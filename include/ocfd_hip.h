@@ -215,12 +215,14 @@ int ocfd_batch_box(ocfd_batch *b, int32_t i, ocfd_handle **h);
  * repaired (error if a box has more than 10 in one stage). */
 int ocfd_batch_step(ocfd_batch *b, int32_t nsteps);
 /* Autotuner: per scheme group and axis, the fastest sweep tile shape of the
- * batched launch (overwrites the boxes' du). */
+ * batched launch, and the segment length (4, 8, 16 or 32 planes per
+ * workgroup) of the k-marching viscous kernels (overwrites the boxes' du and
+ * Ev work arrays; results do not depend on either choice). */
 int ocfd_batch_autotune(ocfd_batch *b, int32_t reps);
 int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape[3]);
-/* Number of scheme groups, their tile shapes (3 per group; shapes may be
- * NULL) and the total cell count. */
-int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells);
+/* Number of scheme groups, their tile shapes (3 per group), the total cell
+ * count and the marching segment length (any pointer may be NULL). */
+int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells, int32_t *kseg);
 
 /* ---- instrumentation -------------------------------------------------- */
 /* Enable per-kernel HIP-event timing on the handle's stream. */

@@ -131,6 +131,7 @@ int set_params(ocfd_handle *h) {
     if (c.scheme_boundary[q] < -1 || c.scheme_boundary[q] > 1) return fail("Scheme_boundary must be -1, 0 or 1");
   Block &B = h->B;
   B.prec = c.precision == OCFD_PREC_MIXED ? OCFD_PREC_MIXED_CODE : OCFD_PREC_FP64_CODE;
+  B.kseg = 0;  // set by a batch
   Geom &g = B.g;
   g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
   g.sx = c.nx + 2 * LAP; g.sy = c.ny + 2 * LAP; g.sz = c.nz + 2 * LAP;
@@ -858,8 +859,9 @@ struct ocfd_batch {
   struct ViscGroup {          // boxes sharing IF_Viscous and the CD order
     bool visc = false, cd8 = false;
     std::vector<int> ids;
-    Launch l;
+    Launch l[NBATCH_KSEG];     // per segment length BATCH_KSEGS[q]
   };
+  int kq = NBATCH_KSEG - 1;    // chosen segment length: BATCH_KSEGS[kq]
   std::vector<SweepGroup> sweep;
   std::vector<ViscGroup> vis;
   Launch halo;
@@ -929,12 +931,14 @@ int batch_step_body(ocfd_batch *b) {
         if (batch_sweep(b, g, a, g.shape[a])) return -1;
     for (const auto &v : b->vis) {
       if (!v.visc) continue;
-      const hipError_t e = launch_visc_flux_batch(boxset(b, v.l), b->prec, v.cd8, (int)v.l.ntiles, b->st);
+      const auto &l = v.l[b->kq];
+      const hipError_t e = launch_visc_flux_batch(boxset(b, l), b->prec, v.cd8, (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched viscous flux: ") + hipGetErrorString(e));
     }
     for (const auto &v : b->vis) {
-      const hipError_t e = launch_visc_div_update_batch(boxset(b, v.l), b->prec, v.visc, v.cd8, krk, b->neg,
-                                                        (int)v.l.ntiles, b->st);
+      const auto &l = v.l[b->kq];
+      const hipError_t e = launch_visc_div_update_batch(boxset(b, l), b->prec, v.visc, v.cd8, krk, b->neg,
+                                                        (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched stage update: ") + hipGetErrorString(e));
     }
     const hipError_t e = launch_negt_repair_batch(b->dblk, (int)b->box.size(), b->neg, b->st);
@@ -969,6 +973,7 @@ int batch_sync_tables(ocfd_batch *b) {
   for (ocfd_handle *h : b->box) {
     HIPCHK(hipStreamSynchronize(h->st));
     blk.push_back(h->B);
+    blk.back().kseg = BATCH_KSEGS[b->kq];
   }
   HIPCHK(hipMemcpy(b->dblk, blk.data(), blk.size() * sizeof(Block), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dtab, b->tab.data(), b->tab.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1037,7 +1042,9 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
     for (int a = 0; a < 3; a++)
       for (int sh = 0; sh < NSWEEP_SHAPES; sh++)
         g.l[a][sh] = add_launch(b, g.ids, [a, sh](const Geom &gg) { return sweep_tiles(gg, a, sh); });
-  for (auto &v : b->vis) v.l = add_launch(b, v.ids, [](const Geom &g) { return march_tiles(g, batch_kseg(g)); });
+  for (auto &v : b->vis)
+    for (int q = 0; q < NBATCH_KSEG; q++)
+      v.l[q] = add_launch(b, v.ids, [q](const Geom &g) { return march_tiles(g, BATCH_KSEGS[q]); });
   int64_t maxt = 0;
   for (int x : b->tab) maxt = std::max<int64_t>(maxt, x);
   if (maxt >= INT32_MAX / 2) {
@@ -1108,7 +1115,8 @@ int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape
   return 0;
 }
 
-int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells) {
+int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells, int32_t *kseg) {
+  if (kseg) *kseg = BATCH_KSEGS[b->kq];
   if (ngroups) *ngroups = (int)b->sweep.size();
   if (shapes)
     for (size_t g = 0; g < b->sweep.size(); g++)
@@ -1144,6 +1152,44 @@ int ocfd_batch_autotune(ocfd_batch *b, int32_t reps) {
       }
       g.shape[a] = bests;
     }
+  }
+  // segment length of the k-marching kernels, timed on the viscous flux
+  // kernel (it writes only Ev); the descriptors are re-sent per candidate
+  bool anyvisc = false;
+  for (const auto &v : b->vis) anyvisc |= v.visc;
+  if (!rc && anyvisc) {
+    float best = 0.f;
+    int bestq = b->kq;
+    for (int q = 0; q < NBATCH_KSEG && !rc; q++) {
+      b->kq = q;
+      if (batch_sync_tables(b)) {
+        rc = -1;
+        break;
+      }
+      float tot = 0.f;
+      for (const auto &v : b->vis) {
+        if (!v.visc) continue;
+        const auto &l = v.l[q];
+        hipError_t e = launch_visc_flux_batch(boxset(b, l), b->prec, v.cd8, (int)l.ntiles, b->st);
+        e = e != hipSuccess ? e : hipEventRecord(e0, b->st);
+        for (int r = 0; r < reps && e == hipSuccess; r++)
+          e = launch_visc_flux_batch(boxset(b, l), b->prec, v.cd8, (int)l.ntiles, b->st);
+        float ms = 0.f;
+        e = e != hipSuccess ? e : hipEventRecord(e1, b->st);
+        e = e != hipSuccess ? e : hipEventSynchronize(e1);
+        e = e != hipSuccess ? e : hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail(std::string("autotune: ") + hipGetErrorString(e));
+          break;
+        }
+        tot += ms;
+      }
+      if (!rc && (q == 0 || tot < best)) {
+        best = tot;
+        bestq = q;
+      }
+    }
+    b->kq = bestq;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -22,6 +22,7 @@ struct Block {
   int *hlist[3], *hmap[3], *hcount;
   double *wflux[3];
   int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
+  int kseg;  // boxes of a batch: planes per workgroup of the k-marching kernels
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
@@ -135,11 +136,12 @@ hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bo
                                         int ntiles, hipStream_t st);
 hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st);
 hipError_t launch_negt_repair_batch(const Block *blk, int nbox, int *neg, hipStream_t st);
-// planes per workgroup of the k-marching kernels for a box of a batch, and
-// their workgroup count (the batch fills the chip with many boxes, so each
-// box is marched in long segments)
-int batch_kseg(const Geom &g);
+// workgroups of the k-marching kernels of a block marched in segments of
+// kseg planes (a batch fills the chip with many boxes; its autotuner picks
+// the segment length from BATCH_KSEGS)
 int64_t march_tiles(const Geom &g, int kseg);
+constexpr int NBATCH_KSEG = 4;
+constexpr int BATCH_KSEGS[NBATCH_KSEG] = {4, 8, 16, 32};
 // workgroups of the periodic halo fill of one box
 int64_t halo_tiles(const Geom &g);
 hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st);

@@ -369,6 +369,8 @@ template <int AXIS, int SCH, bool CHAR, int TL, int TW, typename R>
 __global__ __launch_bounds__(NTHREADS) void k_inviscid_batch(const BoxSet S, const int mode) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int j = S.entry(t);
+  // the descriptor is read in place (a local copy would live in scratch:
+  // the sweep indexes some of its arrays at run time)
   sweep_tile<AXIS, SCH, CHAR, TL, TW, true, R>(S.blk[S.ids[j]], t - S.off[j], mode);
 }
 

@@ -514,16 +514,18 @@ __global__ __launch_bounds__(NT) void k_visc_div_update(const Block B, const int
 }
 
 // Batches of periodic boxes with uniform metrics (ocfd_batch): the tiles of
-// all boxes of the launch in XCD order; box b marches segments of
-// batch_kseg(g) planes and counts its T <= 0 cells in neg + NEG_STRIDE * b.
-__device__ inline int batch_kseg_dev(const Geom &g) { return g.nz < KSEG_MAX ? g.nz : KSEG_MAX; }
+// all boxes of the launch in XCD order; box b marches segments of B.kseg
+// planes (at most nz) and counts its T <= 0 cells in neg + NEG_STRIDE * b.
+__device__ inline int batch_kseg_dev(const Block &B) { return B.g.nz < B.kseg ? B.g.nz : B.kseg; }
 
 template <bool CD8, typename RT>
 __global__ __launch_bounds__(NT) void k_visc_flux_batch(const BoxSet S) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int j = S.entry(t);
-  const Block &B = S.blk[S.ids[j]];
-  visc_flux_tile<CD8, true, RT>(B, batch_kseg_dev(B.g), t - S.off[j]);
+  // copied before any store, so its fields load once into scalar registers
+  // (read in place after the loop's stores they would take vector registers)
+  const Block B = S.blk[S.ids[j]];
+  visc_flux_tile<CD8, true, RT>(B, batch_kseg_dev(B), t - S.off[j]);
 }
 
 template <bool VISC, bool CD8, typename RT>
@@ -531,8 +533,8 @@ __global__ __launch_bounds__(NT) void k_visc_div_update_batch(const BoxSet S, co
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int j = S.entry(t);
   const int box = S.ids[j];
-  const Block &B = S.blk[box];
-  visc_div_update_tile<VISC, CD8, true, RT>(B, krk, 1, batch_kseg_dev(B.g), neg + NEG_STRIDE * box, t - S.off[j]);
+  const Block B = S.blk[box];  // copied before any store, as above
+  visc_div_update_tile<VISC, CD8, true, RT>(B, krk, 1, batch_kseg_dev(B), neg + NEG_STRIDE * box, t - S.off[j]);
 }
 
 // ---------------------------------------------------------------------------

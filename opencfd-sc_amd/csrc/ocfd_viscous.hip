@@ -39,7 +39,6 @@ hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bo
   return visc_div_update_batch_as<double>(S, visc, cd8, krk, neg, ntiles, st);
 }
 
-int batch_kseg(const Geom &g) { return g.nz < KSEG_MAX ? g.nz : KSEG_MAX; }
-int64_t march_tiles(const Geom &g, int kseg) { return march_blocks(g, kseg); }
+int64_t march_tiles(const Geom &g, int kseg) { return march_blocks(g, kseg < g.nz ? kseg : g.nz); }
 
 }  // namespace ocfd

@@ -350,21 +350,25 @@ class Batch:
         return self._call("step", ctypes.c_int32(nsteps))
 
     def autotune(self, reps=5):
+        """Tile shapes per scheme group and the marching segment length."""
         self._call("autotune", ctypes.c_int32(reps))
-        return self.info()[1]
+        info = self.info()
+        return {"sweep_tiles": info[1], "kseg": info[3]}
 
     def set_sweep_tiles(self, group, shapes):
         a = (ctypes.c_int32 * 3)(*[int(x) for x in shapes])
         return self._call("set_sweep_tiles", ctypes.c_int32(group), a)
 
     def info(self):
-        """(number of scheme groups, their tile shapes, total cells)"""
+        """(number of scheme groups, their tile shapes, total cells, marching
+        segment length)"""
         ng = ctypes.c_int32()
         cells = ctypes.c_int64()
-        self._call("info", ctypes.byref(ng), None, ctypes.byref(cells))
+        kseg = ctypes.c_int32()
+        self._call("info", ctypes.byref(ng), None, ctypes.byref(cells), None)
         sh = (ctypes.c_int32 * (3 * max(ng.value, 1)))()
-        self._call("info", ctypes.byref(ng), sh, ctypes.byref(cells))
-        return ng.value, [list(sh[3 * g:3 * g + 3]) for g in range(ng.value)], cells.value
+        self._call("info", ctypes.byref(ng), sh, ctypes.byref(cells), ctypes.byref(kseg))
+        return ng.value, [list(sh[3 * g:3 * g + 3]) for g in range(ng.value)], cells.value, kseg.value
 
     def synchronize(self):
         for b in self.boxes:

@@ -66,7 +66,7 @@ def test_batch_equals_single_boxes(precision):
     with native.Batch([c.cfg for c in cs]) as batch:
         for i, c in enumerate(cs):
             native.setup_block(batch.box(i), c, use_uniform=True)
-        ngroups, shapes, cells = batch.info()
+        ngroups, shapes, cells, kseg = batch.info()
         assert ngroups == 5 and cells == sum(c.cfg.nx * c.cfg.ny * c.cfg.nz for c in cs)
         assert batch.step(3) == 0
         check_equal(batch, ref3, "3 batched steps")
