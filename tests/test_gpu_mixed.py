@@ -93,8 +93,12 @@ def test_mixed_isotropic_steps_close_to_fp64(scheme):
 
 
 def test_mixed_against_oracle():
-    """The same bound against the oracle itself (non-periodic closures, field metrics)."""
-    case = cases.random_case(shape=(20, 28, 40), seed=77, scheme="WENO7", split=2, char=1)
+    """The same bound against the oracle itself, on a curvilinear block with
+    field metrics, non-periodic in y (closures on one side, ghost cells on
+    the other): the general kernels rather than the periodic ones."""
+    case = cases.random_case(shape=(20, 28, 40), seed=77, scheme="WENO7", split=2, char=1, grid="curv",
+                             iperiodic=(1, 0, 1), scheme_boundary=(0, 0, 1, 0, 0, 0),
+                             ghost_cell=(0, 0, 2, 0, 0, 0), bc="BC_USER_DEF", bc_para={1: 1.05})
     ob = pyoracle.setup_oracle(case)
     with block(case, PREC_MIXED) as bmx:
         ob.step(3)
