@@ -132,6 +132,18 @@ int ocfd_download_f(ocfd_handle *h, double *f);
 int ocfd_download_fn(ocfd_handle *h, double *fn);
 int ocfd_download_du(ocfd_handle *h, double *du);
 int ocfd_upload_du(ocfd_handle *h, const double *du);
+/* Snapshot / restart files in the reference's format: save_flow_data
+ * (OCFD_IO.f90:44-64) writes record (Istep, tt) then, for d, u, v, w, T, one
+ * record per k plane of nx*ny doubles; path NULL names the file
+ * OCFDnnnnnnnn.dat after the handle's Istep.  with_f != 0 appends f(:,:,k,m)
+ * records (the reference's reader stops before them), so that a restart
+ * continues bit for bit.  read_flow_data (OCFD_IO.f90:5-39; path NULL =
+ * "opencfd.dat") replaces the interior of the primitives, sets Istep and tt,
+ * and takes f from the file when present (*has_f = 1), else recomputes it
+ * from the primitives as at start-up; call ocfd_bc afterwards, as the
+ * reference's start-up does (OCFD_NS_Solver.f90:15). */
+int ocfd_save_flow_data(ocfd_handle *h, const char *path, int32_t with_f);
+int ocfd_read_flow_data(ocfd_handle *h, const char *path, int32_t *has_f);
 /* Download halo'd primitives (any pointer may be NULL). */
 int ocfd_download_primitives(ocfd_handle *h, double *d, double *u, double *v,
                              double *w, double *T);

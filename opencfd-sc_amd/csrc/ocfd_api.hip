@@ -503,6 +503,115 @@ int ocfd_upload_primitives(ocfd_handle *h, const double *d, const double *u, con
   return 0;
 }
 
+// ---- restart / snapshot files ------------------------------------------------
+// Fortran unformatted sequential records: int32 length, payload, int32 length.
+namespace {
+bool write_rec(FILE *fp, const void *p, int64_t n) {
+  const int32_t len = (int32_t)n;
+  return fwrite(&len, 4, 1, fp) == 1 && fwrite(p, 1, (size_t)n, fp) == (size_t)n && fwrite(&len, 4, 1, fp) == 1;
+}
+// 1: record read; 0: clean end of file before it; -1: malformed
+int read_rec(FILE *fp, void *p, int64_t n) {
+  int32_t a = 0, b = 0;
+  const size_t got = fread(&a, 1, 4, fp);
+  if (got == 0 && feof(fp)) return 0;
+  if (got != 4 || a != (int32_t)n) return -1;
+  if (fread(p, 1, (size_t)n, fp) != (size_t)n || fread(&b, 4, 1, fp) != 1 || b != a) return -1;
+  return 1;
+}
+struct FileCloser {
+  FILE *fp;
+  ~FileCloser() {
+    if (fp) fclose(fp);
+  }
+};
+}  // namespace
+
+int ocfd_save_flow_data(ocfd_handle *h, const char *path, int32_t with_f) {
+  char name[32];
+  if (!path) {
+    std::snprintf(name, sizeof name, "OCFD%08d.dat", h->istep);
+    path = name;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return fail(std::string("cannot open ") + path + " for writing");
+  FileCloser fc{fp};
+  const Geom &g = h->B.g;
+  unsigned char head[12];
+  const int32_t istep = h->istep;
+  std::memcpy(head, &istep, 4);
+  std::memcpy(head + 4, &h->tt, 8);
+  if (!write_rec(fp, head, 12)) return fail("write error");
+  // save_flow_data (OCFD_IO.f90:44-64) / write_3d1: per field, one record per
+  // k plane of the nx x ny interior
+  std::vector<double> buf((size_t)g.nh), plane((size_t)g.nx * g.ny);
+  for (int q = 0; q < 5; q++) {
+    HIPCHK(hipMemcpy(buf.data(), prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+    for (int k = 1; k <= g.nz; k++) {
+      for (int j = 1; j <= g.ny; j++)
+        std::memcpy(&plane[(size_t)(j - 1) * g.nx], &buf[(size_t)g.hi(1, j, k)], (size_t)g.nx * sizeof(double));
+      if (!write_rec(fp, plane.data(), (int64_t)plane.size() * 8)) return fail("write error");
+    }
+  }
+  if (with_f) {
+    // the conserved variables, appended after the reference's records (its
+    // reader stops before them): f(:,:,k,m) per record
+    std::vector<double> f((size_t)5 * g.ni);
+    HIPCHK(hipMemcpy(f.data(), h->B.f, h->f_bytes(), hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < 5 * (int64_t)g.nz; r++)
+      if (!write_rec(fp, &f[(size_t)(r * g.nx * g.ny)], (int64_t)g.nx * g.ny * 8)) return fail("write error");
+  }
+  if (std::fflush(fp) != 0) return fail("write error");
+  return 0;
+}
+
+int ocfd_read_flow_data(ocfd_handle *h, const char *path, int32_t *has_f) {
+  FILE *fp = std::fopen(path ? path : "opencfd.dat", "rb");
+  if (!fp) return fail(std::string("cannot open ") + (path ? path : "opencfd.dat"));
+  FileCloser fc{fp};
+  const Geom &g = h->B.g;
+  unsigned char head[12];
+  if (read_rec(fp, head, 12) != 1) return fail("malformed flow-data file (first record)");
+  int32_t istep;
+  double tt;
+  std::memcpy(&istep, head, 4);
+  std::memcpy(&tt, head + 4, 8);
+  // read_flow_data (OCFD_IO.f90:5-39) / read_3d1: only the interior of each
+  // primitive is replaced; the halos keep their values
+  HIPCHK(hipStreamSynchronize(h->st));
+  std::vector<double> buf((size_t)g.nh), plane((size_t)g.nx * g.ny);
+  for (int q = 0; q < 5; q++) {
+    HIPCHK(hipMemcpy(buf.data(), prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+    for (int k = 1; k <= g.nz; k++) {
+      if (read_rec(fp, plane.data(), (int64_t)plane.size() * 8) != 1)
+        return fail("malformed flow-data file (field " + std::to_string(q + 1) + ", plane " + std::to_string(k) +
+                    "): expected records of nx*ny doubles");
+      for (int j = 1; j <= g.ny; j++)
+        std::memcpy(&buf[(size_t)g.hi(1, j, k)], &plane[(size_t)(j - 1) * g.nx], (size_t)g.nx * sizeof(double));
+    }
+    HIPCHK(hipMemcpy(prims_of(h, q), buf.data(), h->halo_bytes(), hipMemcpyHostToDevice));
+  }
+  h->istep = istep;
+  h->tt = tt;
+  // f from the primitives, as at start-up (OCFD_init.f90), unless the file
+  // carries f
+  std::vector<double> f((size_t)5 * g.ni);
+  int rc = read_rec(fp, f.data(), (int64_t)g.nx * g.ny * 8);
+  if (rc < 0) return fail("malformed conserved-variable records");
+  if (rc == 1) {
+    for (int64_t r = 1; r < 5 * (int64_t)g.nz; r++)
+      if (read_rec(fp, &f[(size_t)(r * g.nx * g.ny)], (int64_t)g.nx * g.ny * 8) != 1)
+        return fail("malformed conserved-variable records");
+    HIPCHK(hipMemcpy(h->B.f, f.data(), h->f_bytes(), hipMemcpyHostToDevice));
+  } else if (launch(h, "init_f", [&] { return launch_init_f(h->B, h->st); })) {
+    return -1;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (has_f) *has_f = rc == 1 ? 1 : 0;
+  return 0;
+}
+
 int ocfd_upload_f(ocfd_handle *h, const double *f) {
   HIPCHK(hipMemcpy(h->B.f, f, h->f_bytes(), hipMemcpyHostToDevice));
   return 0;

@@ -8,7 +8,7 @@
 ! reference's original.  The state lives on the GPU from the first OCFD_bc
 ! (end of initialisation, OCFD_NS_Solver.f90:15) on; host arrays are refreshed
 ! from the device only where host code reads them (show_flow_msg,
-! OCFD_analysis, save_flow_data).
+! OCFD_analysis); save_flow_data is written by the library from the device.
 !
 !   exchange_boundary_xyz(d..T)  -> ocfd_exchange_primitives (OCFD_mpi.f90:183-196)
 !   comput_Rhybrid               -> ocfd_comput_rhybrid      (OCFD_SetHybrid.f90:2-152)
@@ -268,15 +268,25 @@ subroutine wrap_ocfd_analysis() bind(C, name="__wrap_ocfd_analysis_")
    host_section = .false.
 end subroutine wrap_ocfd_analysis
 
+! save_flow_data (OCFD_IO.f90:44-64): the library writes the snapshot from
+! the device state in the reference's record layout, with the reference's
+! file name and messages
 subroutine wrap_save_flow_data() bind(C, name="__wrap_save_flow_data_")
+   use flow_data
+   use ocfd_hip
    use ocfd_dropin_state
    implicit none
    interface
       subroutine real_save_flow_data() bind(C, name="__real_save_flow_data_")
       end subroutine
    end interface
-   if (c_associated(hdev)) call ocfd_dropin_download()
-   host_section = .true.
-   call real_save_flow_data()
-   host_section = .false.
+   character(len=100) :: filename1
+   if (.not. c_associated(hdev)) then
+      call real_save_flow_data()
+      return
+   end if
+   write (filename1, "('OCFD'I8.8'.dat')") Istep
+   if (my_id .eq. 0) print *, 'write data file:', filename1
+   call ocfd_check(ocfd_save_flow_data(hdev, trim(filename1)//c_null_char, 0_c_int32_t), 'ocfd_save_flow_data')
+   if (my_id .eq. 0) print *, 'write data ok'
 end subroutine wrap_save_flow_data

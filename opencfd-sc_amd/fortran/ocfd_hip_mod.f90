@@ -73,6 +73,18 @@ module ocfd_hip
          type(c_ptr), value :: h
          real(c_double), intent(out) :: d(*), u(*), v(*), w(*), t(*)
       end function
+      integer(c_int) function ocfd_save_flow_data(h, path, with_f) bind(C, name="ocfd_save_flow_data")
+         import :: c_int, c_ptr, c_char, c_int32_t
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int32_t), value :: with_f
+      end function
+      integer(c_int) function ocfd_read_flow_data(h, path, has_f) bind(C, name="ocfd_read_flow_data")
+         import :: c_int, c_ptr, c_char, c_int32_t
+         type(c_ptr), value :: h
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int32_t), intent(out) :: has_f
+      end function
       integer(c_int) function ocfd_get_time(h, istep, tt) bind(C, name="ocfd_get_time")
          import :: c_int, c_ptr, c_double, c_int32_t
          type(c_ptr), value :: h

@@ -121,6 +121,18 @@ class BlockBase:
         self._call("download_rhybrid", _ptr(a))
         return a
 
+    # -- restart / snapshot files (save_flow_data / read_flow_data) ---------------
+    def save_flow_data(self, path=None, with_f=False):
+        p = path.encode() if path is not None else None
+        return self._call("save_flow_data", ctypes.c_char_p(p), ctypes.c_int32(1 if with_f else 0))
+
+    def read_flow_data(self, path=None):
+        """Returns True when the file carried f (bit-exact restart)."""
+        has_f = ctypes.c_int32()
+        p = path.encode() if path is not None else None
+        self._call("read_flow_data", ctypes.c_char_p(p), ctypes.byref(has_f))
+        return bool(has_f.value)
+
     def get_time(self):
         i = ctypes.c_int32()
         t = ctypes.c_double()

@@ -14,6 +14,10 @@ reference's outputs:
   met       Ajac, Akx1, Aiy1, Asz1 (halo'd) after init_mesh (curvilinear cases)
   hist      per step: Istep, tt, E, K, S (show_flow_msg), KE, enstrophy (ana_box)
 
+Also writes ref_snapshot_w5_sw.dat: the snapshot file (save_flow_data,
+OCFD_IO.f90:44-64) that the unmodified reference writes after two steps of the
+"w5_sw" case, the fixture of the library's restart-file reader and writer.
+
 Usage (needs the reference built by `make -C oracle ref probe`):
     python tests/golden/make_golden.py [case ...]
 """
@@ -75,12 +79,26 @@ def cfg_to_json(cfg) -> str:
     return json.dumps(d)
 
 
+SNAPSHOT_CASE, SNAPSHOT_STEPS = "w5_sw", 2
+
+
+def snapshot(tmp):
+    case = golden_cases()[SNAPSHOT_CASE]
+    d = os.path.join(tmp, "snapshot")
+    pyoracle.run_reference(case, d, SNAPSHOT_STEPS)
+    path = os.path.join(HERE, "ref_snapshot_%s.dat" % SNAPSHOT_CASE)
+    shutil.copy(os.path.join(d, "OCFD%08d.dat" % SNAPSHOT_STEPS), path)
+    print("%-20s %7.1f KB" % (os.path.basename(path), os.path.getsize(path) / 1024))
+
+
 def main():
     if not pyoracle.reference_available():
         raise SystemExit("build the reference first: make -C oracle ref probe")
     tmp = tempfile.mkdtemp(prefix="ocfd_golden_")
     try:
         only = set(sys.argv[1:])
+        if not only or "snapshot" in only:
+            snapshot(tmp)
         for name, case in golden_cases().items():
             if only and name not in only:
                 continue

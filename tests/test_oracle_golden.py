@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from util import golden_names, load_golden
+from util import GOLDEN, golden_names, load_golden
 
 NAMES = golden_names()
 
@@ -61,3 +61,18 @@ def test_oracle_diagnostics_match_reference_history(name):
         a = ob.ana_box()
         np.testing.assert_allclose(e, row[2:5], rtol=1e-14, atol=0)
         np.testing.assert_allclose([a[12], a[13]], row[5:7], rtol=1e-13, atol=0)
+
+
+def test_reference_snapshot_fixture_layout():
+    """The reference's snapshot file: (Istep, tt) record, then for each of
+    d, u, v, w, T one record per k plane of nx*ny doubles (OCFD_IO.f90:44-64);
+    read with the host reader, it holds the w5_sw case after two steps."""
+    import os
+    from ocfd import fortran_io
+    case, _ = load_golden("w5_sw")
+    path = os.path.join(GOLDEN, "ref_snapshot_w5_sw.dat")
+    nz, ny, nx = case.cfg.shape
+    assert os.path.getsize(path) == (4 + 12 + 4) + 5 * nz * (4 + nx * ny * 8 + 4)
+    istep, tt, prims = fortran_io.read_flow_data(path, case.cfg.shape)
+    assert istep == 2 and abs(tt - 2 * case.cfg.dt) <= 1e-15 * abs(tt)
+    assert all(np.all(np.isfinite(p)) for p in prims) and np.all(prims[0] > 0) and np.all(prims[4] > 0)
