@@ -41,7 +41,7 @@ enum {
     OCFD_SCHEME_HYBRID = 10
 };
 enum { OCFD_SPLIT_SW = 1, OCFD_SPLIT_LLF = 2 };
-enum { OCFD_BC_NONE = 0, OCFD_BC_BLUNT2D = 1, OCFD_BC_USER_DEF = 99 };
+enum { OCFD_BC_NONE = 0, OCFD_BC_BLUNT2D = 1, OCFD_BC_BOUNDARYLAYER = 2, OCFD_BC_SWEPTCORNER = 3, OCFD_BC_USER_DEF = 99 };
 /* Arithmetic of the right-hand side.  OCFD_PREC_MIXED forms the flux
    splitting, the reconstructions of the sweeps and the viscous derivatives,
    stresses and fluxes and accumulates du in FP32; the primitives, f, fn, the
@@ -174,6 +174,31 @@ int ocfd_save_fn(ocfd_handle *h);
 int ocfd_comput_duvwt(ocfd_handle *h);
 /* OCFD_bc (OCFD_BC.f90:2-99) incl. Flow_Ghost_boundary. */
 int ocfd_bc(ocfd_handle *h);
+/* Data of BoundaryCondition = BC_BoundaryLayer (ibc 2, ocfd_bc_BoundaryLayer,
+ * OCFD_BC_Boundarylayers.f90:17-167) or BC_SweptCorner (ibc 3,
+ * ocfd_bc_swept_corner, :443-617), which the reference prepares on the host
+ * in init_bcdata_boundarylayer / init_bc_nonRef / init_bc_wall_perturbation
+ * (BC%flow_inlet*, BC%flow_upper*, BC%NonReflect_upper, BC%wall_pertb):
+ *   aoa     Para%AoA (radians): free stream (cos aoa, sin aoa, 0)
+ *   inlet   d,u,v,w,T at i = 1 as [5][nz][ny] (1-d data repeated over k);
+ *           NULL for the free-stream inlet (BC_para(1) = 0)
+ *   upper   d,u,v,w,T at j = ny as [5][nz][nx]; NULL for the free stream
+ *   nonref  [nz][nx], 1 = non-reflecting upper point (no Dirichlet value);
+ *           NULL = all Dirichlet
+ *   pertb   [nz][nx] wall blow-and-suction shape; NULL = none
+ *   wall    swept corner only, [nz][nx]: 1 = no-slip wall point, 0 = point
+ *           extrapolated from j = 2 (OCFD_BC_Boundarylayers.f90:517); NULL =
+ *           all wall
+ * The other parameters come from the config's bc_para.  Ghost_Cell = -1 on
+ * j- (boundary layer) / k- (swept corner) selects the user ghost layers of
+ * OCFD_GhostCell_UserDef.f90.  Call before the first ocfd_bc. */
+int ocfd_set_bc_boundarylayer(ocfd_handle *h, double aoa, const double *inlet, const double *upper,
+                              const int32_t *nonref, const double *pertb, const int32_t *wall);
+/* Time factor ht of the wall perturbation for the coming stages
+ * (get_ht_multifrequancy, OCFD_BC_Boundarylayers.f90:401-440, evaluated by the
+ * caller at the step's time; 1 until set).  Steps replayed from a graph read
+ * the current value. */
+int ocfd_set_bc_time_factor(ocfd_handle *h, double ht);
 /* filtering(f) (OCFD_filtering.f90:4-70) at the handle's current Istep. */
 int ocfd_filtering(ocfd_handle *h);
 /* Advance Istep by one and tt by dt (OCFD_NS_Solver.f90:64-65). */

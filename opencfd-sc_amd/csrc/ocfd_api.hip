@@ -53,6 +53,9 @@ struct ocfd_handle {
   bool prof = false;
   bool use_graph = true;           // replay whole steps from a captured hipGraph
   int tile[3] = {0, 0, 0};         // sweep tile shape per axis (SweepTile)
+  BLArgs bl = {};                  // boundary-layer / swept-corner data (ibc 2, 3)
+  bool bl_set = false;
+  std::vector<void *> bl_mem;      // its device arrays
   hipGraphExec_t gexec = nullptr;  // one time step (three stages), captured on first use
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
@@ -123,7 +126,8 @@ int set_params(ocfd_handle *h) {
   if (!valid_scheme(c.scheme_invis)) return fail("Scheme_Invis not supported");
   if (c.scheme_vis != SCH_CD6 && c.scheme_vis != SCH_CD8) return fail("Scheme_Vis must be CD6 or CD8");
   if (c.flux_splitting != 1 && c.flux_splitting != 2) return fail("Flux_Splitting must be 1 or 2");
-  if (c.ibc != 0 && c.ibc != 1 && c.ibc != 99) return fail("BoundaryCondition not supported");
+  if (c.ibc != 0 && c.ibc != 1 && c.ibc != 2 && c.ibc != 3 && c.ibc != 99)
+    return fail("BoundaryCondition not supported");
   if (c.nfiltering < 0 || c.nfiltering > 10) return fail("NFiltering must be <= 10");
   if (c.precision != OCFD_PREC_FP64 && c.precision != OCFD_PREC_MIXED)
     return fail("precision must be OCFD_PREC_FP64 or OCFD_PREC_MIXED");
@@ -425,6 +429,7 @@ int ocfd_destroy(ocfd_handle *h) {
     if (h->met[m]) (void)hipFree(h->met[m]);
   if (h->neg) (void)hipFree(h->neg);
   if (h->counts) (void)hipFree(h->counts);
+  for (void *p : h->bl_mem) (void)hipFree(p);
   for (auto &r : h->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -456,7 +461,7 @@ int ocfd_set_grid(ocfd_handle *h, const double *x, const double *y, const double
   }
   if (!rc) {
     for (int m = 0; m < M_N; m++) (void)fill(h->met[m], g.nh, 1.0, h->st);
-    rc = launch(h, "metrics", [&] { return launch_metrics(h->B, dc, h->met, spans, h->ghost, nullptr, h->st); });
+    rc = launch(h, "metrics", [&] { return launch_metrics(h->B, h->ibc, dc, h->met, spans, h->ghost, nullptr, h->st); });
     if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail("metrics kernels failed");
   }
   for (int c = 0; c < 3; c++)
@@ -714,10 +719,87 @@ int ocfd_comput_duvwt(ocfd_handle *h) {
 }
 
 int ocfd_bc(ocfd_handle *h) {
-  return launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); });
+  if ((h->ibc == 2 || h->ibc == 3) && !h->bl_set)
+    return fail("BC_BoundaryLayer / BC_SweptCorner: call ocfd_set_bc_boundarylayer first");
+  return launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, &h->bl, h->ghost, 1, h->st); });
 }
 
 int ocfd_filtering(ocfd_handle *h) { return filtering(h); }
+
+int ocfd_set_bc_boundarylayer(ocfd_handle *h, double aoa, const double *inlet, const double *upper,
+                              const int32_t *nonref, const double *pertb, const int32_t *wall) {
+  const ocfd_config &c = h->cfg;
+  if (c.ibc != 2 && c.ibc != 3) return fail("ocfd_set_bc_boundarylayer needs BC_BoundaryLayer or BC_SweptCorner");
+  const Geom &g = h->B.g;
+  const int64_t nyz = (int64_t)g.ny * g.nz, nxz = (int64_t)g.nx * g.nz;
+  const double *bp = c.bc_para;
+  BLArgs a = {};
+  int inlet_mode, upper_mode;
+  if (c.ibc == 2) {  // ocfd_bc_BoundaryLayer, BC_para(1:14) (OCFD_BC_Boundarylayers.f90:26-43)
+    inlet_mode = (int)std::lround(bp[0]);
+    upper_mode = (int)std::lround(bp[1]);
+    a.outlet = (int)std::lround(bp[3]);
+    a.Tw = bp[4];
+    a.A = bp[7];
+  } else {  // ocfd_bc_swept_corner, BC_para(1:12) (:452-472): 1-d inlet data, free-stream upper
+    inlet_mode = 1;
+    upper_mode = 0;
+    a.outlet = (int)std::lround(bp[0]);
+    a.Tw = bp[1];
+    a.A = bp[4];
+    a.z1 = (int)std::lround(bp[10]);
+    a.z2 = (int)std::lround(bp[11]);
+  }
+  if (inlet_mode != 0 && !inlet) return fail("this boundary condition reads inlet data: pass inlet[5][nz][ny]");
+  if (upper_mode != 0 && !upper) return fail("this boundary condition reads upper-boundary data: pass upper[5][nz][nx]");
+  a.u0 = std::cos(aoa);
+  a.v0 = std::sin(aoa);
+  a.inletT = 0.0;  // the free-stream inlet sets T = 0 (OCFD_BC_Boundarylayers.f90:59)
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (void *p : h->bl_mem) (void)hipFree(p);
+  h->bl_mem.clear();
+  auto put = [&](const void *src, size_t bytes, const void **dst) -> int {
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    h->bl_mem.push_back(d);
+    if (src) HIPCHK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(d, 0, bytes));
+    *dst = d;
+    return 0;
+  };
+  const void *p;
+  if (inlet_mode != 0) {
+    if (put(inlet, 5 * nyz * sizeof(double), &p)) return -1;
+    a.inlet = (const double *)p;
+  }
+  if (upper_mode != 0) {
+    if (put(upper, 5 * nxz * sizeof(double), &p)) return -1;
+    a.upper = (const double *)p;
+  }
+  if (put(nonref, nxz * sizeof(int32_t), &p)) return -1;
+  a.nonref = (const int *)p;
+  if (put(pertb, nxz * sizeof(double), &p)) return -1;
+  a.pertb = (const double *)p;
+  if (wall) {
+    if (put(wall, nxz * sizeof(int32_t), &p)) return -1;
+    a.wall = (const int *)p;
+  }
+  const double one = 1.0;
+  if (put(&one, sizeof(double), &p)) return -1;
+  a.ht = (const double *)p;
+  h->bl = a;
+  h->bl_set = true;
+  invalidate_graph(h);
+  return 0;
+}
+
+int ocfd_set_bc_time_factor(ocfd_handle *h, double ht) {
+  if (!h->bl_set) return fail("call ocfd_set_bc_boundarylayer first");
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy((void *)h->bl.ht, &ht, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
 
 int ocfd_advance_clock(ocfd_handle *h) {
   h->istep += 1;
@@ -744,11 +826,13 @@ int step_body(ocfd_handle *h) {
       return -1;
     if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
     if (h->ibc != 0) {
-      if (launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, h->ghost, 1, h->st); })) return -1;
+      if ((h->ibc == 2 || h->ibc == 3) && !h->bl_set)
+        return fail("BC_BoundaryLayer / BC_SweptCorner: call ocfd_set_bc_boundarylayer first");
+      if (launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, &h->bl, h->ghost, 1, h->st); })) return -1;
     } else {
       bool any = false;
       for (int q = 0; q < 6; q++) any |= (h->ghost[q] == 1 || h->ghost[q] == 2);
-      if (any && launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, h->ghost, 0, h->st); })) return -1;
+      if (any && launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, nullptr, h->ghost, 0, h->st); })) return -1;
     }
   }
   return 0;

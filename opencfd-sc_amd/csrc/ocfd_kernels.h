@@ -125,6 +125,23 @@ inline int64_t sweep_tiles(const Geom &g, int axis, int shape) {
   return (int64_t)((n + tl - 1) / tl) * ((nw + tw - 1) / tw) * nc;
 }
 
+// Data of the boundary-layer / swept-corner conditions (ocfd_bc_BoundaryLayer,
+// ocfd_bc_swept_corner, OCFD_BC_Boundarylayers.f90): device arrays the
+// caller provides (ocfd_set_bc_boundarylayer) and the parameters taken from
+// BC_para.  inlet [5][nz][ny] and upper [5][nz][nx] (NULL: free stream),
+// nonref [nz][nx] (1: non-reflecting upper point, no Dirichlet value),
+// pertb [nz][nx] wall blow-and-suction shape, wall [nz][nx] (swept corner:
+// 1 = no-slip wall point, 0 = extrapolated; NULL: all wall), ht the current
+// time factor of the perturbation (one value, read by the kernels so that
+// captured steps see updates).
+struct BLArgs {
+  const double *inlet, *upper, *pertb, *ht;
+  const int *nonref, *wall;
+  double u0, v0, Tw, A;  // free-stream direction, wall temperature, perturbation amplitude
+  double inletT;         // T of the free-stream inlet (the reference sets 0)
+  int outlet, z1, z2;    // outlet extrapolation order (0 none), side extrapolation flags
+};
+
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, hipStream_t st);
 hipError_t launch_hybrid_lists(const Block &B, hipStream_t st);
@@ -156,14 +173,15 @@ hipError_t launch_rk3(const Block &B, int krk, hipStream_t st);
 hipError_t launch_copy_f(const Block &B, hipStream_t st);
 hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st);
 hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st);
-hipError_t launch_bc(const Block &B, int ibc, double tw, const int *ghost, int planes, hipStream_t st);
+hipError_t launch_bc(const Block &B, int ibc, double tw, const BLArgs *bl, const int *ghost, int planes,
+                     hipStream_t st);
 hipError_t launch_init_f(const Block &B, hipStream_t st);
 hipError_t launch_filter(const Block &B, int axis, int scheme, double s0, double rth, const int rng[6],
                          double *fout, double *p, hipStream_t st);
 hipError_t launch_reduce_flow_msg(const Block &B, double *partial, double *out, hipStream_t st);
 hipError_t launch_ana_box(const Block &B, double *grad, double *amu, double *partial, double *out,
                           hipStream_t st);
-hipError_t launch_metrics(const Block &B, double *coords[3], double *met[M_N], const double spans[9],
+hipError_t launch_metrics(const Block &B, int ibc, double *coords[3], double *met[M_N], const double spans[9],
                           const int ghost[6], double *grad, hipStream_t st);
 hipError_t launch_normalise_metrics(const Block &B, double *met[M_N], hipStream_t st);
 

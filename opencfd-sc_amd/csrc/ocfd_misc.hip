@@ -314,6 +314,96 @@ __global__ __launch_bounds__(NT) void k_bc_hook(const Block B, const int ibc, co
   B.d[hn] = 1.0; B.u[hn] = 1.0; B.v[hn] = 0.0; B.w[hn] = 0.0; B.T[hn] = 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// boundary-layer / swept-corner conditions (ocfd_bc_BoundaryLayer,
+// OCFD_BC_Boundarylayers.f90:17-167; ocfd_bc_swept_corner, :443-617), one
+// kernel per part, launched in the reference's order: inlet, upper, wall,
+// outlet, spanwise sides (a later part overwrites the edges of an earlier one)
+// ---------------------------------------------------------------------------
+__device__ inline void set_prims(const Block &B, int64_t h, double d, double u, double v, double w, double T) {
+  B.d[h] = d; B.u[h] = u; B.v[h] = v; B.w[h] = w; B.T[h] = T;
+}
+
+// i = 1: free stream, or the inlet data
+__global__ __launch_bounds__(NT) void k_bl_inlet(const Block B, const BLArgs a) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.ny * g.nz) return;
+  const int64_t h = g.hi(1, (int)(t % g.ny) + 1, (int)(t / g.ny) + 1);
+  if (!a.inlet) {
+    set_prims(B, h, 1.0, a.u0, a.v0, 0.0, a.inletT);
+  } else {
+    const int64_t n = (int64_t)g.ny * g.nz;
+    set_prims(B, h, a.inlet[t], a.inlet[t + n], a.inlet[t + 2 * n], a.inlet[t + 3 * n], a.inlet[t + 4 * n]);
+  }
+}
+
+// j = ny: Dirichlet values where the point is not non-reflecting
+__global__ __launch_bounds__(NT) void k_bl_upper(const Block B, const BLArgs a) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.nz || a.nonref[t] != 0) return;
+  const int64_t h = g.hi((int)(t % g.nx) + 1, g.ny, (int)(t / g.nx) + 1);
+  if (!a.upper) {
+    set_prims(B, h, 1.0, a.u0, a.v0, 0.0, 1.0);
+  } else {
+    const int64_t n = (int64_t)g.nx * g.nz;
+    set_prims(B, h, a.upper[t], a.upper[t + n], a.upper[t + 2 * n], a.upper[t + 3 * n], a.upper[t + 4 * n]);
+  }
+}
+
+// j = 1: blowing/suction along the wall normal, isothermal (Tw > 0) or
+// adiabatic wall with second-order pressure; swept corner: points off the
+// wall extrapolate at first order
+__global__ __launch_bounds__(NT) void k_bl_wall(const Block B, const BLArgs a) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.nz) return;
+  const int i = (int)(t % g.nx) + 1, k = (int)(t / g.nx) + 1;
+  const int64_t h1 = g.hi(i, 1, k), h2 = g.hi(i, 2, k), h3 = g.hi(i, 3, k);
+  if (a.wall && a.wall[t] == 0) {
+    set_prims(B, h1, B.d[h2], B.u[h2], B.v[h2], B.w[h2], B.T[h2]);
+    return;
+  }
+  const double aix = B.M.get(M_AIX, h1), aiy = B.M.get(M_AIY, h1), aiz = B.M.get(M_AIZ, h1);
+  const double tmp = 1.0 / (sqrt(aix * aix + aiy * aiy + aiz * aiz));
+  const double A0 = a.A * a.ht[0] * a.pertb[t];
+  B.u[h1] = A0 * aix * tmp;
+  B.v[h1] = A0 * aiy * tmp;
+  B.w[h1] = A0 * aiz * tmp;
+  if (a.Tw > 0.0) {
+    B.T[h1] = a.Tw;
+    B.d[h1] = (4.0 * B.d[h2] * B.T[h2] - B.d[h3] * B.T[h3]) / (3.0 * a.Tw);
+  } else {
+    B.T[h1] = (4.0 * B.T[h2] - B.T[h3]) / 3.0;
+    B.d[h1] = (4.0 * B.d[h2] - B.d[h3]) / 3.0;
+  }
+}
+
+// i = nx: first- or second-order extrapolation
+__global__ __launch_bounds__(NT) void k_bl_outlet(const Block B, const BLArgs a) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.ny * g.nz) return;
+  const int j = (int)(t % g.ny) + 1, k = (int)(t / g.ny) + 1;
+  const int64_t h = g.hi(g.nx, j, k), h1 = g.hi(g.nx - 1, j, k), h2 = g.hi(g.nx - 2, j, k);
+  double *F[5] = {B.d, B.u, B.v, B.w, B.T};
+  for (int q = 0; q < 5; q++) F[q][h] = a.outlet == 1 ? F[q][h1] : 2.0 * F[q][h1] - F[q][h2];
+}
+
+// swept corner, k = 1 / k = nz: first-order extrapolation where requested
+__global__ __launch_bounds__(NT) void k_bl_sides(const Block B, const BLArgs a) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.ny) return;
+  const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
+  double *F[5] = {B.d, B.u, B.v, B.w, B.T};
+  if (a.z1 == 1)
+    for (int q = 0; q < 5; q++) F[q][g.hi(i, j, 1)] = F[q][g.hi(i, j, 2)];
+  if (a.z2 == 1)
+    for (int q = 0; q < 5; q++) F[q][g.hi(i, j, g.nz)] = F[q][g.hi(i, j, g.nz - 1)];
+}
+
 // f recomputed from the primitives on the six block faces (OCFD_BC.f90:132-200)
 __global__ __launch_bounds__(NT) void k_bc_planes(const Block B) {
   const Geom &g = B.g;
@@ -373,6 +463,47 @@ __global__ __launch_bounds__(NT) void k_ghost(const Block B, const int side) {
   if (T > s2 * B.T[wq]) T = s2 * B.T[wq];
   B.d[gq] = d;
   B.T[gq] = T;
+}
+
+// Flow_Ghost_Userdef (OCFD_GhostCell_UserDef.f90:14-21): flat plate j- (the
+// boundary layer; mirror about j = 1 with the 0.5 / 2.0 limiter,
+// Flow_Ghost_flatplate :110-136) or swept-corner symmetry plane k- (w
+// changes sign, Flow_Ghost_Sweptcorner :56-76)
+__global__ __launch_bounds__(NT) void k_ghost_user(const Block B, const int ibc) {
+  const Geom &g = B.g;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (ibc == 2) {
+    if (t >= (int64_t)LAP * g.nx * g.nz) return;
+    const int l = (int)(t % LAP);
+    const int64_t r = t / LAP;
+    const int i = (int)(r % g.nx) + 1, k = (int)(r / g.nx) + 1;
+    const int j = 1 - LAP + l, j1 = 2 - j;
+    const int64_t h = g.hi(i, j, k), hw = g.hi(i, 1, k), hm = g.hi(i, j1, k);
+    const double s1 = 0.5, s2 = 2.0;
+    B.u[h] = 2.0 * B.u[hw] - B.u[hm];
+    B.v[h] = 2.0 * B.v[hw] - B.v[hm];
+    B.w[h] = 2.0 * B.w[hw] - B.w[hm];
+    double d = 2.0 * B.d[hw] - B.d[hm];
+    double T = 2.0 * B.T[hw] - B.T[hm];
+    if (d < s1 * B.d[hw]) d = s1 * B.d[hw];
+    if (d > s2 * B.d[hw]) d = s2 * B.d[hw];
+    if (T < s1 * B.T[hw]) T = s1 * B.T[hw];
+    if (T > s2 * B.T[hw]) T = s2 * B.T[hw];
+    B.d[h] = d;
+    B.T[h] = T;
+  } else {
+    if (t >= (int64_t)LAP * g.nx * g.ny) return;
+    const int l = (int)(t % LAP);
+    const int64_t r = t / LAP;
+    const int i = (int)(r % g.nx) + 1, j = (int)(r / g.nx) + 1;
+    const int k = 1 - LAP + l, k1 = 1 - k;
+    const int64_t h = g.hi(i, j, k), hm = g.hi(i, j, k1);
+    B.u[h] = B.u[hm];
+    B.v[h] = B.v[hm];
+    B.w[h] = -B.w[hm];
+    B.d[h] = B.d[hm];
+    B.T[h] = B.T[hm];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -672,6 +803,42 @@ __global__ __launch_bounds__(NT) void k_jac_ghost2(const Block B, const int side
   jac_point(mp.p, h, dd[0][0], dd[0][1], dd[0][2], dd[1][0], dd[1][1], dd[1][2], dd[2][0], dd[2][1], dd[2][2]);
 }
 
+// Jac_Ghost_Userdef (OCFD_GhostCell_UserDef.f90:4-12): the metrics and
+// coordinates of the flat-plate j- ghost layers mirrored about j = 1
+// (Jac_Ghost_flatplate :79-106) or of the swept-corner k- layers about the
+// plane between k = 0 and 1 (Jac_Ghost_Sweptcorner :25-52), with the
+// components normal to the plane negated
+__global__ __launch_bounds__(NT) void k_jac_ghost_user(const Geom g, const int ibc, double *X, double *Y, double *Z,
+                                                       const MetPtrs mp) {
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  int64_t h, hm;
+  // sign of Akx Aky Akz Aix Aiy Aiz Asx Asy Asz Ajac, then X Y Z
+  double sg[13];
+  for (int q = 0; q < 13; q++) sg[q] = 1.0;
+  if (ibc == 2) {
+    if (t >= (int64_t)LAP * g.nx * g.nz) return;
+    const int l = (int)(t % LAP);
+    const int64_t r = t / LAP;
+    const int i = (int)(r % g.nx) + 1, k = (int)(r / g.nx) + 1;
+    const int j = 1 - LAP + l;
+    h = g.hi(i, j, k);
+    hm = g.hi(i, 2 - j, k);
+    sg[1] = sg[3] = sg[5] = sg[7] = sg[11] = -1.0;  // Aky Aix Aiz Asy, Y
+  } else {
+    if (t >= (int64_t)LAP * g.nx * g.ny) return;
+    const int l = (int)(t % LAP);
+    const int64_t r = t / LAP;
+    const int i = (int)(r % g.nx) + 1, j = (int)(r / g.nx) + 1;
+    const int k = 1 - LAP + l;
+    h = g.hi(i, j, k);
+    hm = g.hi(i, j, 1 - k);
+    sg[2] = sg[5] = sg[6] = sg[7] = sg[12] = -1.0;  // Akz Aiz Asx Asy, Z
+  }
+  for (int q = 0; q < 10; q++) mp.p[q][h] = sg[q] < 0.0 ? -mp.p[q][hm] : mp.p[q][hm];
+  double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; c++) C[c][h] = sg[10 + c] < 0.0 ? -C[c][hm] : C[c][hm];
+}
+
 // Akx1 = Akx/Ajac ... over the whole halo'd block (OCFD_mesh.f90:41-55)
 __global__ __launch_bounds__(NT) void k_normalise(const Geom g, const MetPtrs mp) {
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -762,9 +929,19 @@ hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_bc(const Block &B, int ibc, double tw, const int *ghost, int planes, hipStream_t st) {
+hipError_t launch_bc(const Block &B, int ibc, double tw, const BLArgs *bl, const int *ghost, int planes,
+                     hipStream_t st) {
   const Geom &g = B.g;
   if (ibc == 1 || ibc == 99) hipLaunchKernelGGL(k_bc_hook, dim3(nb((int64_t)g.nx * g.nz)), dim3(NT), 0, st, B, ibc, tw);
+  if ((ibc == 2 || ibc == 3) && bl) {
+    const int64_t nyz = (int64_t)g.ny * g.nz, nxz = (int64_t)g.nx * g.nz;
+    hipLaunchKernelGGL(k_bl_inlet, dim3(nb(nyz)), dim3(NT), 0, st, B, *bl);
+    hipLaunchKernelGGL(k_bl_upper, dim3(nb(nxz)), dim3(NT), 0, st, B, *bl);
+    hipLaunchKernelGGL(k_bl_wall, dim3(nb(nxz)), dim3(NT), 0, st, B, *bl);
+    if (bl->outlet == 1 || bl->outlet == 2) hipLaunchKernelGGL(k_bl_outlet, dim3(nb(nyz)), dim3(NT), 0, st, B, *bl);
+    if (ibc == 3 && (bl->z1 == 1 || bl->z2 == 1))
+      hipLaunchKernelGGL(k_bl_sides, dim3(nb((int64_t)g.nx * g.ny)), dim3(NT), 0, st, B, *bl);
+  }
   const int64_t np = 2 * ((int64_t)g.ny * g.nz + (int64_t)g.nx * g.nz + (int64_t)g.nx * g.ny);
   if (planes) hipLaunchKernelGGL(k_bc_planes, dim3(nb(np)), dim3(NT), 0, st, B);
   for (int side = 0; side < 6; side++) {
@@ -773,6 +950,12 @@ hipError_t launch_bc(const Block &B, int ibc, double tw, const int *ghost, int p
     const int64_t cnt = (int64_t)LAP * (axis == 0 ? (int64_t)g.ny * g.nz : (axis == 1 ? (int64_t)g.nx * g.nz : (int64_t)g.nx * g.ny));
     hipLaunchKernelGGL(k_ghost, dim3(nb(cnt)), dim3(NT), 0, st, B, side);
   }
+  // user ghost layers (Ghost_Cell = -1): flat plate j- / swept corner k-
+  const int ibc_any = ibc < 0 ? -ibc : ibc;
+  if (ibc_any == 2 && ghost[2] == -1)
+    hipLaunchKernelGGL(k_ghost_user, dim3(nb((int64_t)LAP * g.nx * g.nz)), dim3(NT), 0, st, B, 2);
+  if (ibc_any == 3 && ghost[4] == -1)
+    hipLaunchKernelGGL(k_ghost_user, dim3(nb((int64_t)LAP * g.nx * g.ny)), dim3(NT), 0, st, B, 3);
   return hipGetLastError();
 }
 
@@ -814,7 +997,7 @@ hipError_t launch_ana_box(const Block &B, double *grad, double *amu, double *par
   return hipGetLastError();
 }
 
-hipError_t launch_metrics(const Block &B, double *coords[3], double *met[M_N], const double spans[9],
+hipError_t launch_metrics(const Block &B, int ibc, double *coords[3], double *met[M_N], const double spans[9],
                           const int ghost[6], double *grad, hipStream_t st) {
   (void)grad;
   const Geom &g = B.g;
@@ -846,6 +1029,13 @@ hipError_t launch_metrics(const Block &B, double *coords[3], double *met[M_N], c
     hipLaunchKernelGGL(k_jac_ghost2, dim3(nb(cnt)), dim3(NT), 0, st, B, side, (const double *)coords[0],
                        (const double *)coords[1], (const double *)coords[2], mp);
   }
+  // Jac_Ghost_Userdef after the extrapolated layers
+  if (ibc == 2 && ghost[2] == -1)
+    hipLaunchKernelGGL(k_jac_ghost_user, dim3(nb((int64_t)LAP * g.nx * g.nz)), dim3(NT), 0, st, g, 2, coords[0],
+                       coords[1], coords[2], mp);
+  if (ibc == 3 && ghost[4] == -1)
+    hipLaunchKernelGGL(k_jac_ghost_user, dim3(nb((int64_t)LAP * g.nx * g.ny)), dim3(NT), 0, st, g, 3, coords[0],
+                       coords[1], coords[2], mp);
   return launch_normalise_metrics(B, met, st);
 }
 

@@ -11,7 +11,7 @@
 //               primitive recovery, the boundary-plane refresh of f and the
 //               T <= 0 trap (whole-step path; du is not written back)
 //
-// k_visc_flux and k_visc_div_update march along k over a 32x8 tile of (i,j)
+// k_visc_flux and k_visc_div_update march along k over a 64x4 tile of (i,j)
 // columns and a segment of up to 32 planes (fewer on small blocks so the
 // launch still fills the chip).  The x and y neighbours of the current plane
 // come from a double-buffered LDS tile: while one plane is computed, every
@@ -61,7 +61,7 @@ __device__ inline R di(const Block &B, const R *a, int axis, int i, int j, int k
   return dxyz0_point(at, s, n, B.P.dc[axis], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[axis] == 1);
 }
 
-constexpr int TXT = 32, TYT = 8, KSEG_MAX = 32;
+constexpr int TXT = 64, TYT = 4, KSEG_MAX = 32;
 
 // Plane tile geometry: stencil radius R of the central difference, tile of
 // LX x LY points around the TXT x TYT columns.

@@ -84,8 +84,83 @@ subroutine ocfd_dropin_attach()
    call ocfd_check(ocfd_upload_primitives(hdev, d, u, v, w, T, int(Istep, c_int32_t), tt), &
                    'ocfd_upload_primitives')
    call ocfd_check(ocfd_upload_f(hdev, f), 'ocfd_upload_f')
+   if (Para%IBC == BC_BoundaryLayer .or. Para%IBC == BC_SweptCorner) call ocfd_dropin_bc_data()
    if (my_id .eq. 0) print *, 'libocfd_hip: block on GPU', dev
 end subroutine ocfd_dropin_attach
+
+! The data of ocfd_bc_BoundaryLayer / ocfd_bc_swept_corner that the first
+! (host) OCFD_bc prepared in module BC_data (OCFD_BC_Boundarylayers.f90:1-15,
+! 171-397), handed to the library; 1-d inlet/upper data repeated over k.
+subroutine ocfd_dropin_bc_data()
+   use flow_data
+   use BC_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   real(c_double), allocatable, target :: inl(:, :, :), upp(:, :, :), pert(:, :)
+   integer(c_int32_t), allocatable, target :: nonref(:, :), wmask(:, :)
+   type(c_ptr) :: p_inl, p_upp, p_wall
+   integer :: inlet_mode, upper_mode, j, k, i, m
+   real(c_double), parameter :: epsl = 1.d-7
+
+   if (Para%IBC == BC_BoundaryLayer) then
+      inlet_mode = nint(Para%BC_para(1))
+      upper_mode = nint(Para%BC_para(2))
+   else
+      inlet_mode = 1
+      upper_mode = 0
+   end if
+   p_inl = c_null_ptr
+   p_upp = c_null_ptr
+   p_wall = c_null_ptr
+   if (inlet_mode == 1 .or. inlet_mode == 2) then
+      allocate (inl(ny, nz, 5))
+      do m = 1, 5
+      do k = 1, nz
+      do j = 1, ny
+         if (inlet_mode == 1) then
+            inl(j, k, m) = BC%flow_inlet1d(j, m)
+         else
+            inl(j, k, m) = BC%flow_inlet2d(j, k, m)
+         end if
+      end do
+      end do
+      end do
+      p_inl = c_loc(inl)
+   end if
+   if (upper_mode == 1 .or. upper_mode == 2) then
+      allocate (upp(nx, nz, 5))
+      do m = 1, 5
+      do k = 1, nz
+      do i = 1, nx
+         if (upper_mode == 1) then
+            upp(i, k, m) = BC%flow_upper1d(i, m)
+         else
+            upp(i, k, m) = BC%flow_upper2d(i, k, m)
+         end if
+      end do
+      end do
+      end do
+      p_upp = c_loc(upp)
+   end if
+   allocate (nonref(nx, nz), pert(nx, nz))
+   nonref = BC%NonReflect_upper
+   pert = BC%wall_pertb
+   if (Para%IBC == BC_SweptCorner) then
+      ! no-slip wall points of ocfd_bc_swept_corner (OCFD_BC_Boundarylayers.f90:517)
+      allocate (wmask(nx, nz))
+      do k = 1, nz
+      do i = 1, nx
+         wmask(i, k) = 0
+         if ((Azz(i, 1, k) >= -epsl .and. Azz(i, 1, k) <= Para%Bc_para(3) + epsl) .or. Ayy(i, 1, k) <= epsl) &
+            wmask(i, k) = 1
+      end do
+      end do
+      p_wall = c_loc(wmask)
+   end if
+   call ocfd_check(ocfd_set_bc_boundarylayer(hdev, Para%AoA, p_inl, p_upp, c_loc(nonref), c_loc(pert), p_wall), &
+                   'ocfd_set_bc_boundarylayer')
+end subroutine ocfd_dropin_bc_data
 
 ! Device state -> host arrays d,u,v,w,T (halo'd) and f.
 subroutine ocfd_dropin_download()
@@ -98,6 +173,7 @@ subroutine ocfd_dropin_download()
 end subroutine ocfd_dropin_download
 
 subroutine wrap_ocfd_bc() bind(C, name="__wrap_ocfd_bc_")
+   use flow_data
    use ocfd_hip
    use ocfd_dropin_state
    implicit none
@@ -112,9 +188,31 @@ subroutine wrap_ocfd_bc() bind(C, name="__wrap_ocfd_bc_")
    else if (host_section) then
       call real_ocfd_bc()
    else
+      if (Para%IBC == BC_BoundaryLayer .or. Para%IBC == BC_SweptCorner) call ocfd_dropin_bc_time()
       call ocfd_check(ocfd_bc(hdev), 'ocfd_bc')
    end if
 end subroutine wrap_ocfd_bc
+
+! The wall perturbation's time factor at this call, from the reference's own
+! get_ht_multifrequancy (its random phases are drawn on the first, host, call)
+subroutine ocfd_dropin_bc_time()
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   real(kind=OCFD_REAL_KIND) :: ht
+   integer :: mt
+   real(kind=OCFD_REAL_KIND) :: beta
+   if (Para%IBC == BC_BoundaryLayer) then
+      mt = nint(Para%Bc_para(11))
+      beta = Para%Bc_para(14)
+   else
+      mt = nint(Para%Bc_para(8))
+      beta = Para%Bc_para(10)
+   end if
+   call get_ht_multifrequancy(ht, tt, mt, beta)
+   call ocfd_check(ocfd_set_bc_time_factor(hdev, real(ht, c_double)), 'ocfd_set_bc_time_factor')
+end subroutine ocfd_dropin_bc_time
 
 subroutine wrap_exchange_boundary_xyz(a) bind(C, name="__wrap_exchange_boundary_xyz_")
    use flow_data

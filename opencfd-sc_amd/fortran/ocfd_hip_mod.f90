@@ -85,6 +85,18 @@ module ocfd_hip
          character(kind=c_char), intent(in) :: path(*)
          integer(c_int32_t), intent(out) :: has_f
       end function
+      integer(c_int) function ocfd_set_bc_boundarylayer(h, aoa, inlet, upper, nonref, pertb, wall) &
+         bind(C, name="ocfd_set_bc_boundarylayer")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), value :: aoa
+         type(c_ptr), value :: inlet, upper, nonref, pertb, wall
+      end function
+      integer(c_int) function ocfd_set_bc_time_factor(h, ht) bind(C, name="ocfd_set_bc_time_factor")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), value :: ht
+      end function
       integer(c_int) function ocfd_get_time(h, istep, tt) bind(C, name="ocfd_get_time")
          import :: c_int, c_ptr, c_double, c_int32_t
          type(c_ptr), value :: h

@@ -69,6 +69,14 @@ class Case:
             if cfg.grid_type == GRID_AND_JACOBIAN3D:
                 fields += [np.full(cfg.shape, v) for v in self.uniform_metrics]
             fortran_io.write_grid_3d(gp, *fields)
+        if "inlet1d" in self.info:
+            # 1-d inlet profile of the boundary-layer conditions: a header line,
+            # then y, d, u, v, T per j (init_bcdata_boundarylayer,
+            # OCFD_BC_Boundarylayers.f90:186-196)
+            with open(os.path.join(directory, "flow1d-inlet.dat"), "w") as fh:
+                fh.write("variables = y d u v T\n")
+                for row in self.info["inlet1d"]:
+                    fh.write(" ".join("%.17e" % v for v in row) + "\n")
 
 
 # ---------------------------------------------------------------------------
@@ -379,6 +387,87 @@ def blunt_ogrid(nx=512, ny=256, nz=8, r0=1.0, dr0=None, ratio=1.05, zspan=0.1, m
     case = Case(cfg=cfg, x=X, y=Y, z=Z, spans=np.array(cfg.periodic_span), prims=[d, u, v, w, T],
                 grid_payload=(x2, y2, z1), name="blunt%dx%dx%d" % (nx, ny, nz),
                 info={"dr0": dr0, "ratio": ratio, "t_wall": t_wall, "zspan": zspan})
+    cfg.dt = cfl_dt(case, cfl)
+    return case
+
+
+def _wall_layer(yw, delta, t_wall):
+    """Laminar-like profile over a wall at distance yw: u = tanh(y/delta),
+    T from the wall temperature to 1, uniform pressure (d = 1/T)."""
+    u = np.tanh(yw / delta)
+    T = t_wall + (1.0 - t_wall) * u
+    return 1.0 / T, u, T
+
+
+def flat_plate(nx=24, ny=20, nz=8, x0=0.1, lx=1.0, y_top=0.25, rise=0.15, zspan=0.2, ma=2.25, re=1.0e4,
+               t_wall=1.5, delta=0.03, cfl=0.4, scheme="WENO5", split=1, char=1) -> Case:
+    """Supersonic flat-plate boundary layer with BoundaryCondition =
+    BC_BoundaryLayer (ocfd_bc_BoundaryLayer, OCFD_BC_Boundarylayers.f90:17-167):
+    1-d inlet profile file, free-stream upper boundary whose non-reflecting
+    points are chosen from its slope (flat over the first half, rising by
+    `rise` over the second), isothermal wall with multi-frequency
+    blow-and-suction (random phases, drawn by the reference), second-order
+    outlet, user ghost layers at j- (Ghost_Cell(3) = -1), periodic in z.
+    Grid: GRID2D_PLANE, wall-clustered in y."""
+    xs = x0 + lx * np.arange(nx) / (nx - 1)
+    s = np.arange(ny) / (ny - 1)
+    eta = (np.exp(2.0 * s) - 1.0) / (np.exp(2.0) - 1.0)          # wall clustering
+    top = y_top + rise * np.clip((xs - x0 - 0.5 * lx) / (0.5 * lx), 0.0, None)
+    x2 = np.broadcast_to(xs[None, :], (ny, nx)).copy()
+    y2 = eta[:, None] * top[None, :]
+    z1 = zspan * np.arange(nz) / nz
+    shape = (nz, ny, nx)
+    X = np.broadcast_to(x2, shape).copy()
+    Y = np.broadcast_to(y2, shape).copy()
+    Z = np.broadcast_to(z1[:, None, None], shape).copy()
+    d, u, T = _wall_layer(Y, delta, t_wall)
+    v = np.zeros(shape)
+    w = np.zeros(shape)
+    inlet = np.stack([y2[:, 0], d[0, :, 0], u[0, :, 0], v[0, :, 0], T[0, :, 0]], axis=1)
+    bc = {1: 1, 2: 0, 3: 0, 4: 2, 5: t_wall, 6: 0.0, 7: 1, 8: 0.02, 9: x0 + 0.2 * lx, 10: x0 + 0.7 * lx,
+          11: 3, 12: 2, 13: zspan, 14: 40.0}
+    cfg = CaseConfig(nx=nx, ny=ny, nz=nz, iperiodic=(0, 0, 1), scheme_invis=scheme, scheme_vis="CD6",
+                     flux_splitting=split, if_scheme_character=char, ma=ma, re=re,
+                     grid_type=GRID2D_PLANE, boundary_condition="BC_BOUNDARYLAYER", bc_para=bc,
+                     scheme_boundary=(0, 0, 1, 0, 0, 0), ghost_cell=(0, 0, -1, 0, 0, 0),
+                     periodic_span=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, zspan))
+    case = Case(cfg=cfg, x=X, y=Y, z=Z, spans=np.array(cfg.periodic_span), prims=[d, u, v, w, T],
+                grid_payload=(x2, y2, z1), name="flatplate%dx%dx%d" % (nx, ny, nz),
+                info={"inlet1d": inlet, "t_wall": t_wall})
+    cfg.dt = cfl_dt(case, cfl)
+    return case
+
+
+def swept_corner(nx=24, ny=18, nz=12, x0=0.0, lx=1.0, y_top=0.3, ramp=0.15, zspan=0.3, wall_zl=0.18,
+                 ma=2.5, re=1.0e4, t_wall=1.4, delta=0.03, cfl=0.4, scheme="WENO5", split=1, char=1) -> Case:
+    """Swept compression ramp with BoundaryCondition = BC_SweptCorner
+    (ocfd_bc_swept_corner, OCFD_BC_Boundarylayers.f90:443-617): 1-d inlet
+    profile file, free-stream upper boundary, a wall that is no-slip for
+    z <= wall_zl or where it is flat and extrapolated elsewhere, random
+    wall blowing (type 2, drawn by the reference), second-order outlet,
+    first-order extrapolation at both spanwise sides, symmetry ghost plane at
+    k- (Ghost_Cell(5) = -1).  Grid: GRID3D, the ramp starting mid-plate."""
+    xs = x0 + lx * np.arange(nx) / (nx - 1)
+    s = np.arange(ny) / (ny - 1)
+    eta = (np.exp(2.0 * s) - 1.0) / (np.exp(2.0) - 1.0)
+    zs = zspan * np.arange(nz) / (nz - 1)
+    yw = ramp * np.clip((xs - x0 - 0.5 * lx) / (0.5 * lx), 0.0, None)
+    shape = (nz, ny, nx)
+    X = np.broadcast_to(xs[None, None, :], shape).copy()
+    Y = np.broadcast_to(yw[None, None, :] + eta[None, :, None] * (y_top - yw[None, None, :]), shape).copy()
+    Z = np.broadcast_to(zs[:, None, None], shape).copy()
+    d, u, T = _wall_layer(Y - yw[None, None, :], delta, t_wall)
+    v = np.zeros(shape)
+    w = np.zeros(shape)
+    inlet = np.stack([Y[0, :, 0], d[0, :, 0], u[0, :, 0], v[0, :, 0], T[0, :, 0]], axis=1)
+    bc = {1: 2, 2: t_wall, 3: wall_zl, 4: 2, 5: 0.01, 6: x0 + 0.1 * lx, 7: x0 + 0.6 * lx, 8: 2, 9: 0, 10: 30.0,
+          11: 1, 12: 1}
+    cfg = CaseConfig(nx=nx, ny=ny, nz=nz, iperiodic=(0, 0, 0), scheme_invis=scheme, scheme_vis="CD6",
+                     flux_splitting=split, if_scheme_character=char, ma=ma, re=re,
+                     grid_type=GRID3D, boundary_condition="BC_SWEPTCORNER", bc_para=bc,
+                     scheme_boundary=(0, 0, 0, 0, 1, 0), ghost_cell=(0, 0, 0, 0, -1, 0))
+    case = Case(cfg=cfg, x=X, y=Y, z=Z, spans=np.zeros(9), prims=[d, u, v, w, T],
+                name="swept%dx%dx%d" % (nx, ny, nz), info={"inlet1d": inlet, "t_wall": t_wall})
     cfg.dt = cfl_dt(case, cfl)
     return case
 

@@ -13,7 +13,7 @@ import dataclasses
 from dataclasses import dataclass, field
 
 SCHEMES = {"WENO5": 1, "WENO7": 2, "OMP6": 3, "UD7L": 4, "CD6": 5, "CD8": 6, "HYBRID": 10}
-BCS = {"BC_NONE": 0, "BC_BLUNT2D": 1, "BC_USER_DEF": 99}
+BCS = {"BC_NONE": 0, "BC_BLUNT2D": 1, "BC_BOUNDARYLAYER": 2, "BC_SWEPTCORNER": 3, "BC_USER_DEF": 99}
 GRID_BOX, GRID1D, GRID2D_PLANE, GRID3D, GRID_AND_JACOBIAN3D = 0, 10, 20, 30, 31
 PREC_FP64, PREC_MIXED = 0, 1
 METRIC_FIELD, METRIC_UNIFORM = 0, 1

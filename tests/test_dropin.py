@@ -16,15 +16,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DROPIN = os.path.join(ROOT, "opencfd-sc_amd", "fortran", "_build", "opencfd_hip")
 FC_MOD = os.path.join(ROOT, "opencfd-sc_amd", "fortran", "ocfd_hip_mod.f90")
 
-# (golden case name, steps): periodic hybrid, curvilinear closures, filters,
-# blunt-body BC with mass force, pressure-gradient sensor
+# (case name, steps): periodic hybrid, curvilinear closures, filters,
+# blunt-body BC with mass force, pressure-gradient sensor (the golden cases),
+# then the boundary-layer and swept-corner conditions with their inlet files,
+# time-dependent random wall blowing and user ghost layers
 CASES = [("hyb_c_llf", 3), ("curv_w5c", 3), ("grid1d_shockfilter", 4), ("blunt_massforce", 3),
-         ("hyb_sensor2_curv", 3)]
+         ("hyb_sensor2_curv", 3), ("flat_plate", 4), ("swept_corner", 4)]
 
 
 def _case(name):
-    """The seeded case behind golden vector `name` (same generator)."""
+    """The seeded case behind golden vector `name` (same generator), or a
+    generated boundary-layer case."""
     import sys
+    from ocfd import cases
+    if name == "flat_plate":
+        return cases.flat_plate()
+    if name == "swept_corner":
+        return cases.swept_corner()
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from make_golden import golden_cases
     return golden_cases()[name]
