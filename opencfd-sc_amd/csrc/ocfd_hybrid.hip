@@ -23,7 +23,9 @@ namespace ocfd {
 namespace {
 
 constexpr int NT = 256;
-constexpr int FB = 48;  // listed faces per workgroup batch
+constexpr int FB = 51;   // listed faces per workgroup batch: 5 * FB rows fill the 256 lanes
+constexpr int SFS = 81;  // LDS stride of one face's split fluxes (8 points x 10, padded so
+                         // that the faces a wavefront reads fall in different banks)
 
 // A workgroup lists LPT * NT consecutive faces of the axis: scheme index from
 // the sensor, the WENO faces appended to the axis list in face order, and the
@@ -92,7 +94,7 @@ template <int AXIS, bool CHAR, bool FAST>
 __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   typedef TileIndex<AXIS> TI;
   constexpr int NH = CHAR ? 5 : 10;
-  __shared__ double sF[FB][8][10];  // fp(1:5), fm(1:5) at stencil points s-3 .. s+4
+  __shared__ double sF[FB][SFS];    // fp(1:5), fm(1:5) at stencil points s-3 .. s+4: [p * 10 + m]
   __shared__ double sQ[FB][2][7];   // u, v, w, c, A1..A3 at points s, s+1
   __shared__ double sB[FB][21];     // characteristic basis of the face
   __shared__ double sG[FB][5];      // characteristic fluxes h0(1:5)
@@ -128,8 +130,8 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
       else split_llf(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
 #pragma unroll
       for (int m = 0; m < 5; m++) {
-        sF[fb][p][m] = fp[m];
-        sF[fb][p][5 + m] = fm[m];
+        sF[fb][p * 10 + m] = fp[m];
+        sF[fb][p * 10 + 5 + m] = fm[m];
       }
       if (CHAR && (p == 3 || p == 4)) {
         double *q = sQ[fb][p - 3];
@@ -179,15 +181,15 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         double xp[8], xm[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          xp[k] = sr5[0] * sF[fb][k][0];
-          xm[k] = sr5[0] * sF[fb][k][5];
+          xp[k] = sr5[0] * sF[fb][k * 10];
+          xm[k] = sr5[0] * sF[fb][k * 10 + 5];
         }
 #pragma unroll
         for (int cmp = 1; cmp < 5; cmp++) {
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            xp[k] = xp[k] + sr5[cmp] * sF[fb][k][cmp];
-            xm[k] = xm[k] + sr5[cmp] * sF[fb][k][5 + cmp];
+            xp[k] = xp[k] + sr5[cmp] * sF[fb][k * 10 + cmp];
+            xm[k] = xm[k] + sr5[cmp] * sF[fb][k * 10 + 5 + cmp];
           }
         }
 #pragma unroll
@@ -200,14 +202,14 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         sG[fb][r] = h1 + h2;
       }
       __syncthreads();
-      // back to conservative components (S1 * h0)
-      for (int fb = tid; fb < nf; fb += NT) {
+      // back to conservative components (S1 * h0): one (face, component)
+      // per lane, so the stores of a wavefront are contiguous
+      for (int it = tid; it < nf * 5; it += NT) {
+        const int fb = it / 5, m = it - 5 * fb;
         const CharBasis Bs = basis(fb);
         double hh[5];
         char_back(Bs, sG[fb], hh);
-        double *o = out + (int64_t)(f0 + fb) * NH;
-#pragma unroll
-        for (int m = 0; m < 5; m++) o[m] = hh[m];
+        out[(int64_t)(f0 + fb) * NH + m] = hh[m];
       }
     } else {
       // one component per lane: flux+ and flux- of the face
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         double xp[8], xm[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          xp[k] = sF[fb][k][m];
-          xm[k] = sF[fb][k][5 + m];
+          xp[k] = sF[fb][k * 10 + m];
+          xm[k] = sF[fb][k * 10 + 5 + m];
         }
         double *o = out + (int64_t)(f0 + fb) * NH;
         o[m] = FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_nonchar<SCH_HYBRID>(S, xp, s, n, AXIS, scm);

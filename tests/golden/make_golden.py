@@ -16,7 +16,10 @@ reference's outputs:
 
 Also writes ref_snapshot_w5_sw.dat: the snapshot file (save_flow_data,
 OCFD_IO.f90:44-64) that the unmodified reference writes after two steps of the
-"w5_sw" case, the fixture of the library's restart-file reader and writer.
+"w5_sw" case, the fixture of the library's restart-file reader and writer;
+and met_flat_plate.npz / met_swept_corner.npz: the halo'd metrics Ajac, Akx1,
+Aiy1, Asz1 that the reference's init_mesh computes for the boundary-layer
+cases (their user ghost layers, Jac_Ghost_Userdef).
 
 Usage (needs the reference built by `make -C oracle ref probe`):
     python tests/golden/make_golden.py [case ...]
@@ -91,6 +94,15 @@ def snapshot(tmp):
     print("%-20s %7.1f KB" % (os.path.basename(path), os.path.getsize(path) / 1024))
 
 
+def bc_metrics(tmp):
+    for name, mk in (("flat_plate", cases.flat_plate), ("swept_corner", cases.swept_corner)):
+        case = mk()
+        ref = pyoracle.run_probe(case, os.path.join(tmp, "met_" + name), nextra=0)
+        path = os.path.join(HERE, "met_%s.npz" % name)
+        np.savez_compressed(path, met=np.stack([ref[k] for k in ("Ajac", "Akx1", "Aiy1", "Asz1")]))
+        print("%-20s %7.1f KB" % (os.path.basename(path), os.path.getsize(path) / 1024))
+
+
 def main():
     if not pyoracle.reference_available():
         raise SystemExit("build the reference first: make -C oracle ref probe")
@@ -99,6 +111,8 @@ def main():
         only = set(sys.argv[1:])
         if not only or "snapshot" in only:
             snapshot(tmp)
+        if not only or "bc_metrics" in only:
+            bc_metrics(tmp)
         for name, case in golden_cases().items():
             if only and name not in only:
                 continue

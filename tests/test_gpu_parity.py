@@ -280,3 +280,20 @@ def test_autotune_keeps_results():
         ref.step(2)
         b.step(2)
         assert_same(b.download_f(), ref.download_f(), "f after autotune + 2 steps (tiles %s)" % shapes)
+
+
+@pytest.mark.parametrize("name", ["flat_plate", "swept_corner"])
+def test_boundary_layer_metrics_with_user_ghosts(name):
+    """Metrics computed on the device for the boundary-layer cases, whose
+    Ghost_Cell = -1 sides take the user ghost layers (Jac_Ghost_Userdef,
+    OCFD_GhostCell_UserDef.f90:4-106), equal the reference's init_mesh output
+    (tests/golden/met_<name>.npz) bit for bit, ghost layers included."""
+    import os
+    from util import GOLDEN
+    case = getattr(cases, name)()
+    ref = np.load(os.path.join(GOLDEN, "met_%s.npz" % name), allow_pickle=False)["met"]
+    with native.Block(case.cfg) as b:
+        b.set_grid(case.x, case.y, case.z, case.spans)
+        mets = b.download_metrics()
+    for q, m in enumerate((9, 10, 14, 18)):  # Ajac, Akx1, Aiy1, Asz1
+        assert_same(mets[m], ref[q], "metric %d" % m)
