@@ -101,6 +101,21 @@ struct Phys {
 };
 
 // ---------------------------------------------------------------------------
+// madd(a, b, c) = a*b + c: two roundings in FP64 (the reference's), one fused
+// multiply-add in FP32 (the mixed-precision form; explicit, so its rounding
+// does not depend on the surrounding code).  A left-to-right sum of products
+// p0 + p1 + ... written as madd(.., .., madd(.., .., p0)) rounds exactly as
+// the original in FP64, IEEE addition being commutative.
+// ---------------------------------------------------------------------------
+__device__ inline double madd(double a, double b, double c) { return a * b + c; }
+__device__ inline float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// a / b for the nonlinear weights of the WENO schemes: IEEE division in FP64;
+// in FP32 a product with the hardware reciprocal (within 1 ulp), which stays
+// far inside the mixed form's accuracy, set by its FP32 fluxes
+__device__ inline double wdiv(double a, double b) { return a / b; }
+__device__ inline float wdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// ---------------------------------------------------------------------------
 // point-wise reconstructions.  x[k] holds the stencil value v(k-3), k=0..7,
 // i.e. x[3] is the left cell of the face (v(0) in OCFD_Schemes_1.f90).
 // ---------------------------------------------------------------------------
@@ -128,20 +143,20 @@ __device__ inline T weno5p(const T *x) {
   typedef K<T> C;
   const T vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
   T t, S0, S1, S2;
-  t = v0 - T(2.0) * v1 + v2; S0 = C::c1312 * (t * t);
-  t = T(3.0) * v0 - T(4.0) * v1 + v2; S0 = S0 + C::c14 * (t * t);
-  t = vm1 - T(2.0) * v0 + v1; S1 = C::c1312 * (t * t);
-  t = vm1 - v1; S1 = S1 + C::c14 * (t * t);
-  t = vm2 - T(2.0) * vm1 + v0; S2 = C::c1312 * (t * t);
-  t = vm2 - T(4.0) * vm1 + T(3.0) * v0; S2 = S2 + C::c14 * (t * t);
-  T a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  T a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
+  t = madd(T(-2.0), v1, v0) + v2; S0 = C::c1312 * (t * t);
+  t = madd(T(-4.0), v1, T(3.0) * v0) + v2; S0 = madd(C::c14, t * t, S0);
+  t = madd(T(-2.0), v0, vm1) + v1; S1 = C::c1312 * (t * t);
+  t = vm1 - v1; S1 = madd(C::c14, t * t, S1);
+  t = madd(T(-2.0), vm1, vm2) + v0; S2 = C::c1312 * (t * t);
+  t = madd(T(3.0), v0, madd(T(-4.0), vm1, vm2)); S2 = madd(C::c14, t * t, S2);
+  T a0 = wdiv(C::w5c0, (C::w5ep + S0) * (C::w5ep + S0));
+  T a1 = wdiv(C::w5c1, (C::w5ep + S1) * (C::w5ep + S1));
+  T a2 = wdiv(C::w5c2, (C::w5ep + S2) * (C::w5ep + S2));
   T am = a0 + a1 + a2;
-  T q03 = C::c13 * v0 + C::c56 * v1 - C::c16 * v2;
-  T q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
-  T q23 = C::c13 * vm2 - C::c76 * vm1 + C::c116 * v0;
-  return (a0 * q03 + a1 * q13 + a2 * q23) / am;
+  T q03 = madd(-C::c16, v2, madd(C::c56, v1, C::c13 * v0));
+  T q13 = madd(C::c13, v1, madd(C::c56, v0, -(C::c16 * vm1)));
+  T q23 = madd(C::c116, v0, madd(-C::c76, vm1, C::c13 * vm2));
+  return wdiv(madd(a2, q23, madd(a1, q13, a0 * q03)), am);
 }
 
 // WENO5-JS, flux- (hh_weno5M, OCFD_Schemes_1.f90:31-52)
@@ -150,20 +165,20 @@ __device__ inline T weno5m(const T *x) {
   typedef K<T> C;
   const T vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
   T t, S0, S1, S2;
-  t = v1 - T(2.0) * v0 + vm1; S0 = C::c1312 * (t * t);
-  t = T(3.0) * v1 - T(4.0) * v0 + vm1; S0 = S0 + C::c14 * (t * t);
-  t = v2 - T(2.0) * v1 + v0; S1 = C::c1312 * (t * t);
-  t = v2 - v0; S1 = S1 + C::c14 * (t * t);
-  t = v3 - T(2.0) * v2 + v1; S2 = C::c1312 * (t * t);
-  t = v3 - T(4.0) * v2 + T(3.0) * v1; S2 = S2 + C::c14 * (t * t);
-  T a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  T a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
+  t = madd(T(-2.0), v0, v1) + vm1; S0 = C::c1312 * (t * t);
+  t = madd(T(-4.0), v0, T(3.0) * v1) + vm1; S0 = madd(C::c14, t * t, S0);
+  t = madd(T(-2.0), v1, v2) + v0; S1 = C::c1312 * (t * t);
+  t = v2 - v0; S1 = madd(C::c14, t * t, S1);
+  t = madd(T(-2.0), v2, v3) + v1; S2 = C::c1312 * (t * t);
+  t = madd(T(3.0), v1, madd(T(-4.0), v2, v3)); S2 = madd(C::c14, t * t, S2);
+  T a0 = wdiv(C::w5c0, (C::w5ep + S0) * (C::w5ep + S0));
+  T a1 = wdiv(C::w5c1, (C::w5ep + S1) * (C::w5ep + S1));
+  T a2 = wdiv(C::w5c2, (C::w5ep + S2) * (C::w5ep + S2));
   T am = a0 + a1 + a2;
-  T q03 = C::c13 * v1 + C::c56 * v0 - C::c16 * vm1;
-  T q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
-  T q23 = C::c13 * v3 - C::c76 * v2 + C::c116 * v1;
-  return (a0 * q03 + a1 * q13 + a2 * q23) / am;
+  T q03 = madd(-C::c16, vm1, madd(C::c56, v0, C::c13 * v1));
+  T q13 = madd(C::c13, v0, madd(C::c56, v1, -(C::c16 * v2)));
+  T q23 = madd(C::c116, v1, madd(-C::c76, v2, C::c13 * v3));
+  return wdiv(madd(a2, q23, madd(a1, q13, a0 * q03)), am);
 }
 
 // WENO5 with the out-of-domain sub-stencil removed (hh_weno5P_boundary,
@@ -174,23 +189,23 @@ __device__ inline T weno5p_boundary(const T *x, int del) {
   const T vm2 = x[1], vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5];
   T t, S0, S1, S2, a0 = T(0.0), q03 = T(0.0), a2 = T(0.0), q23 = T(0.0);
   if (del != DEL_RIGHT) {
-    t = v0 - T(2.0) * v1 + v2; S0 = C::c1312 * (t * t);
-    t = T(3.0) * v0 - T(4.0) * v1 + v2; S0 = S0 + C::c14 * (t * t);
-    a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-    q03 = C::c13 * v0 + C::c56 * v1 - C::c16 * v2;
+    t = madd(T(-2.0), v1, v0) + v2; S0 = C::c1312 * (t * t);
+    t = madd(T(-4.0), v1, T(3.0) * v0) + v2; S0 = madd(C::c14, t * t, S0);
+    a0 = wdiv(C::w5c0, (C::w5ep + S0) * (C::w5ep + S0));
+    q03 = madd(-C::c16, v2, madd(C::c56, v1, C::c13 * v0));
   }
-  t = vm1 - T(2.0) * v0 + v1; S1 = C::c1312 * (t * t);
-  t = vm1 - v1; S1 = S1 + C::c14 * (t * t);
-  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  T q13 = -(C::c16 * vm1) + C::c56 * v0 + C::c13 * v1;
+  t = madd(T(-2.0), v0, vm1) + v1; S1 = C::c1312 * (t * t);
+  t = vm1 - v1; S1 = madd(C::c14, t * t, S1);
+  T a1 = wdiv(C::w5c1, (C::w5ep + S1) * (C::w5ep + S1));
+  T q13 = madd(C::c13, v1, madd(C::c56, v0, -(C::c16 * vm1)));
   if (del != DEL_LIFT) {
-    t = vm2 - T(2.0) * vm1 + v0; S2 = C::c1312 * (t * t);
-    t = vm2 - T(4.0) * vm1 + T(3.0) * v0; S2 = S2 + C::c14 * (t * t);
-    a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
-    q23 = C::c13 * vm2 - C::c76 * vm1 + C::c116 * v0;
+    t = madd(T(-2.0), vm1, vm2) + v0; S2 = C::c1312 * (t * t);
+    t = madd(T(3.0), v0, madd(T(-4.0), vm1, vm2)); S2 = madd(C::c14, t * t, S2);
+    a2 = wdiv(C::w5c2, (C::w5ep + S2) * (C::w5ep + S2));
+    q23 = madd(C::c116, v0, madd(-C::c76, vm1, C::c13 * vm2));
   }
   T am = a0 + a1 + a2;
-  return (a0 * q03 + a1 * q13 + a2 * q23) / am;
+  return wdiv(madd(a2, q23, madd(a1, q13, a0 * q03)), am);
 }
 
 // hh_weno5M_boundary (OCFD_Schemes_1.f90:297-325)
@@ -200,23 +215,23 @@ __device__ inline T weno5m_boundary(const T *x, int del) {
   const T vm1 = x[2], v0 = x[3], v1 = x[4], v2 = x[5], v3 = x[6];
   T t, S0, S1, S2, a0 = T(0.0), q03 = T(0.0), a2 = T(0.0), q23 = T(0.0);
   if (del != DEL_LIFT) {
-    t = v1 - T(2.0) * v0 + vm1; S0 = C::c1312 * (t * t);
-    t = T(3.0) * v1 - T(4.0) * v0 + vm1; S0 = S0 + C::c14 * (t * t);
-    a0 = C::w5c0 / ((C::w5ep + S0) * (C::w5ep + S0));
-    q03 = C::c13 * v1 + C::c56 * v0 - C::c16 * vm1;
+    t = madd(T(-2.0), v0, v1) + vm1; S0 = C::c1312 * (t * t);
+    t = madd(T(-4.0), v0, T(3.0) * v1) + vm1; S0 = madd(C::c14, t * t, S0);
+    a0 = wdiv(C::w5c0, (C::w5ep + S0) * (C::w5ep + S0));
+    q03 = madd(-C::c16, vm1, madd(C::c56, v0, C::c13 * v1));
   }
-  t = v2 - T(2.0) * v1 + v0; S1 = C::c1312 * (t * t);
-  t = v2 - v0; S1 = S1 + C::c14 * (t * t);
-  T a1 = C::w5c1 / ((C::w5ep + S1) * (C::w5ep + S1));
-  T q13 = -(C::c16 * v2) + C::c56 * v1 + C::c13 * v0;
+  t = madd(T(-2.0), v1, v2) + v0; S1 = C::c1312 * (t * t);
+  t = v2 - v0; S1 = madd(C::c14, t * t, S1);
+  T a1 = wdiv(C::w5c1, (C::w5ep + S1) * (C::w5ep + S1));
+  T q13 = madd(C::c13, v0, madd(C::c56, v1, -(C::c16 * v2)));
   if (del != DEL_RIGHT) {
-    t = v3 - T(2.0) * v2 + v1; S2 = C::c1312 * (t * t);
-    t = v3 - T(4.0) * v2 + T(3.0) * v1; S2 = S2 + C::c14 * (t * t);
-    a2 = C::w5c2 / ((C::w5ep + S2) * (C::w5ep + S2));
-    q23 = C::c13 * v3 - C::c76 * v2 + C::c116 * v1;
+    t = madd(T(-2.0), v2, v3) + v1; S2 = C::c1312 * (t * t);
+    t = madd(T(3.0), v1, madd(T(-4.0), v2, v3)); S2 = madd(C::c14, t * t, S2);
+    a2 = wdiv(C::w5c2, (C::w5ep + S2) * (C::w5ep + S2));
+    q23 = madd(C::c116, v1, madd(-C::c76, v2, C::c13 * v3));
   }
   T am = a0 + a1 + a2;
-  return (a0 * q03 + a1 * q13 + a2 * q23) / am;
+  return wdiv(madd(a2, q23, madd(a1, q13, a0 * q03)), am);
 }
 
 // WENO7-JS core on seven values ordered from the upwind end (OCFD_Schemes.f90:59-193:
@@ -237,32 +252,32 @@ __device__ inline T weno7_core(T m3, T m2, T m1, T c0, T p1, T p2,
   constexpr T e31 = -T(1.0) / T(12.0), e32 = T(7.0) / T(12.0), e33 = T(7.0) / T(12.0), e34 = -T(1.0) / T(12.0);
   constexpr T e41 = T(3.0) / T(12.0), e42 = T(13.0) / T(12.0), e43 = -T(5.0) / T(12.0), e44 = T(1.0) / T(12.0);
   constexpr T ep = T(1.0e-8);
-  T S10 = a11 * m3 + a12 * m2 + a13 * m1 + a14 * c0;
-  T S11 = a21 * m2 - m1 + a23 * c0 + a24 * p1;
-  T S12 = a31 * m1 + a32 * c0 + p1 + a34 * p2;
-  T S13 = a41 * c0 + a42 * p1 + a43 * p2 + a44 * p3;
-  T S20 = -m3 + b12 * m2 + b13 * m1 + b14 * c0;
-  T S21 = m1 + b22 * c0 + p1;
-  T S22 = c0 + b22 * p1 + p2;
-  T S23 = b41 * c0 + b42 * p1 + b43 * p2 - p3;
-  T S30 = -m3 + c12 * (m2 - m1) + c0;
-  T S31 = -m2 + c12 * (m1 - c0) + p1;
-  T S32 = -m1 + c12 * (c0 - p1) + p2;
-  T S33 = -c0 + c12 * (p1 - p2) + p3;
-  T S0 = S10 * S10 + d12 * S20 * S20 + d13 * S30 * S30 + d14 * S10 * S30;
-  T S1 = S11 * S11 + d12 * S21 * S21 + d13 * S31 * S31 + d14 * S11 * S31;
-  T S2 = S12 * S12 + d12 * S22 * S22 + d13 * S32 * S32 + d14 * S12 * S32;
-  T S3 = S13 * S13 + d12 * S23 * S23 + d13 * S33 * S33 + d14 * S13 * S33;
-  T a0 = C0 / ((ep + S0) * (ep + S0));
-  T a1 = C1 / ((ep + S1) * (ep + S1));
-  T a2 = C2 / ((ep + S2) * (ep + S2));
-  T a3 = C3 / ((ep + S3) * (ep + S3));
+  T S10 = madd(a14, c0, madd(a13, m1, madd(a12, m2, a11 * m3)));
+  T S11 = madd(a24, p1, madd(a23, c0, madd(a21, m2, -m1)));
+  T S12 = madd(a34, p2, madd(a32, c0, a31 * m1) + p1);
+  T S13 = madd(a44, p3, madd(a43, p2, madd(a42, p1, a41 * c0)));
+  T S20 = madd(b14, c0, madd(b13, m1, madd(b12, m2, -m3)));
+  T S21 = madd(b22, c0, m1) + p1;
+  T S22 = madd(b22, p1, c0) + p2;
+  T S23 = madd(b43, p2, madd(b42, p1, b41 * c0)) - p3;
+  T S30 = madd(c12, m2 - m1, -m3) + c0;
+  T S31 = madd(c12, m1 - c0, -m2) + p1;
+  T S32 = madd(c12, c0 - p1, -m1) + p2;
+  T S33 = madd(c12, p1 - p2, -c0) + p3;
+  T S0 = madd(d14 * S10, S30, madd(d13 * S30, S30, madd(d12 * S20, S20, S10 * S10)));
+  T S1 = madd(d14 * S11, S31, madd(d13 * S31, S31, madd(d12 * S21, S21, S11 * S11)));
+  T S2 = madd(d14 * S12, S32, madd(d13 * S32, S32, madd(d12 * S22, S22, S12 * S12)));
+  T S3 = madd(d14 * S13, S33, madd(d13 * S33, S33, madd(d12 * S23, S23, S13 * S13)));
+  T a0 = wdiv(C0, (ep + S0) * (ep + S0));
+  T a1 = wdiv(C1, (ep + S1) * (ep + S1));
+  T a2 = wdiv(C2, (ep + S2) * (ep + S2));
+  T a3 = wdiv(C3, (ep + S3) * (ep + S3));
   T am = a0 + a1 + a2 + a3;
-  T q0 = e11 * m3 + e12 * m2 + e13 * m1 + e14 * c0;
-  T q1 = e21 * m2 + e22 * m1 + e23 * c0 + e24 * p1;
-  T q2 = e31 * m1 + e32 * c0 + e33 * p1 + e34 * p2;
-  T q3 = e41 * c0 + e42 * p1 + e43 * p2 + e44 * p3;
-  return (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / am;
+  T q0 = madd(e14, c0, madd(e13, m1, madd(e12, m2, e11 * m3)));
+  T q1 = madd(e24, p1, madd(e23, c0, madd(e22, m1, e21 * m2)));
+  T q2 = madd(e34, p2, madd(e33, p1, madd(e32, c0, e31 * m1)));
+  T q3 = madd(e44, p3, madd(e43, p2, madd(e42, p1, e41 * c0)));
+  return wdiv(madd(a3, q3, madd(a2, q2, madd(a1, q1, a0 * q0))), am);
 }
 template <typename T>
 __device__ inline T weno7p(const T *x) { return weno7_core(x[0], x[1], x[2], x[3], x[4], x[5], x[6]); }
@@ -329,13 +344,17 @@ __device__ inline T omp6m(const T *x) {
 // UD7L (OCFD_Schemes.f90:292-316)
 template <typename T>
 __device__ inline T ud7lp(const double *c, const T *x) {
-  return T(c[0]) * x[0] + T(c[1]) * x[1] + T(c[2]) * x[2] + T(c[3]) * x[3] + T(c[4]) * x[4] + T(c[5]) * x[5] +
-         T(c[6]) * x[6] + T(c[7]) * x[7];
+  T s = T(c[0]) * x[0];
+#pragma unroll
+  for (int k = 1; k < 8; k++) s = madd(T(c[k]), x[k], s);
+  return s;
 }
 template <typename T>
 __device__ inline T ud7lm(const double *c, const T *x) {
-  return T(c[0]) * x[7] + T(c[1]) * x[6] + T(c[2]) * x[5] + T(c[3]) * x[4] + T(c[4]) * x[3] + T(c[5]) * x[2] +
-         T(c[6]) * x[1] + T(c[7]) * x[0];
+  T s = T(c[0]) * x[7];
+#pragma unroll
+  for (int k = 1; k < 8; k++) s = madd(T(c[k]), x[7 - k], s);
+  return s;
 }
 
 // set_hybrid_scheme (OCFD_SetHybrid.f90:207-220)
@@ -472,31 +491,31 @@ __device__ inline void split_sw(T gamma, T c1, T d1, T u1, T v1, T w1, T A1,
                                 T A2, T A3, T *fp, T *fm) {
   const T tmp1 = T(2.0) * (gamma - T(1.0));
   const T tmp3 = (T(3.0) - gamma) / (T(2.0) * (gamma - T(1.0)));
-  T ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+  T ss = sqrt(madd(A3, A3, madd(A2, A2, A1 * A1)));
   T ss1 = T(1.0) / ss;
   T ak1 = A1 * ss1, ak2 = A2 * ss1, ak3 = A3 * ss1;
-  T vs = A1 * u1 + A2 * v1 + A3 * w1;
-  T E1 = vs, E2 = vs - c1 * ss, E3 = vs + c1 * ss;
+  T vs = madd(A3, w1, madd(A2, v1, A1 * u1));
+  T E1 = vs, E2 = madd(-c1, ss, vs), E3 = madd(c1, ss, vs);
   T E1P = (E1 + fabs(E1)) * T(0.5), E2P = (E2 + fabs(E2)) * T(0.5), E3P = (E3 + fabs(E3)) * T(0.5);
   T E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
   T tmp0 = d1 / (T(2.0) * gamma);
-  T uc1 = u1 - c1 * ak1, uc2 = u1 + c1 * ak1;
-  T vc1 = v1 - c1 * ak2, vc2 = v1 + c1 * ak2;
-  T wc1 = w1 - c1 * ak3, wc2 = w1 + c1 * ak3;
-  T vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * T(0.5);
-  T vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * T(0.5);
-  T vv = (gamma - T(1.0)) * (u1 * u1 + v1 * v1 + w1 * w1);
+  T uc1 = madd(-c1, ak1, u1), uc2 = madd(c1, ak1, u1);
+  T vc1 = madd(-c1, ak2, v1), vc2 = madd(c1, ak2, v1);
+  T wc1 = madd(-c1, ak3, w1), wc2 = madd(c1, ak3, w1);
+  T vvc1 = madd(wc1, wc1, madd(vc1, vc1, uc1 * uc1)) * T(0.5);
+  T vvc2 = madd(wc2, wc2, madd(vc2, vc2, uc2 * uc2)) * T(0.5);
+  T vv = (gamma - T(1.0)) * madd(w1, w1, madd(v1, v1, u1 * u1));
   T W2 = tmp3 * c1 * c1;
-  fp[0] = tmp0 * (tmp1 * E1P + E2P + E3P);
-  fp[1] = tmp0 * (tmp1 * E1P * u1 + E2P * uc1 + E3P * uc2);
-  fp[2] = tmp0 * (tmp1 * E1P * v1 + E2P * vc1 + E3P * vc2);
-  fp[3] = tmp0 * (tmp1 * E1P * w1 + E2P * wc1 + E3P * wc2);
-  fp[4] = tmp0 * (E1P * vv + E2P * vvc1 + E3P * vvc2 + W2 * (E2P + E3P));
-  fm[0] = tmp0 * (tmp1 * E1M + E2M + E3M);
-  fm[1] = tmp0 * (tmp1 * E1M * u1 + E2M * uc1 + E3M * uc2);
-  fm[2] = tmp0 * (tmp1 * E1M * v1 + E2M * vc1 + E3M * vc2);
-  fm[3] = tmp0 * (tmp1 * E1M * w1 + E2M * wc1 + E3M * wc2);
-  fm[4] = tmp0 * (E1M * vv + E2M * vvc1 + E3M * vvc2 + W2 * (E2M + E3M));
+  fp[0] = tmp0 * (madd(tmp1, E1P, E2P) + E3P);
+  fp[1] = tmp0 * madd(E3P, uc2, madd(E2P, uc1, tmp1 * E1P * u1));
+  fp[2] = tmp0 * madd(E3P, vc2, madd(E2P, vc1, tmp1 * E1P * v1));
+  fp[3] = tmp0 * madd(E3P, wc2, madd(E2P, wc1, tmp1 * E1P * w1));
+  fp[4] = tmp0 * madd(W2, E2P + E3P, madd(E3P, vvc2, madd(E2P, vvc1, E1P * vv)));
+  fm[0] = tmp0 * (madd(tmp1, E1M, E2M) + E3M);
+  fm[1] = tmp0 * madd(E3M, uc2, madd(E2M, uc1, tmp1 * E1M * u1));
+  fm[2] = tmp0 * madd(E3M, vc2, madd(E2M, vc1, tmp1 * E1M * v1));
+  fm[3] = tmp0 * madd(E3M, wc2, madd(E2M, wc1, tmp1 * E1M * w1));
+  fm[4] = tmp0 * madd(W2, E2M + E3M, madd(E3M, vvc2, madd(E2M, vvc1, E1M * vv)));
 }
 
 // split_Local_LaxFriedrichs (OCFD_flux_split.f90:73-104)
@@ -504,22 +523,25 @@ template <typename T>
 __device__ inline void split_llf(T gamma, T c1, T d1, T u1, T v1, T w1,
                                  T A1, T A2, T A3, T *fp, T *fm) {
   const T tmp0 = T(1.0) / gamma, tmp1 = T(1.0) / (gamma - T(1.0));
-  T ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
-  T un = A1 * u1 + A2 * v1 + A3 * w1;
-  T lamda0 = fabs(un) + ss * c1;
+  T ss = sqrt(madd(A3, A3, madd(A2, A2, A1 * A1)));
+  T un = madd(A3, w1, madd(A2, v1, A1 * u1));
+  T lamda0 = madd(ss, c1, fabs(un));
   T p1 = tmp0 * d1 * c1 * c1;
-  T E0 = tmp1 * p1 + T(0.5) * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
+  T E0 = madd(tmp1, p1, T(0.5) * d1 * madd(w1, w1, madd(v1, v1, u1 * u1)));
   T h0 = E0 + p1;
-  fp[0] = T(0.5) * (d1 * un + lamda0 * d1);
-  fp[1] = T(0.5) * (d1 * u1 * un + A1 * p1 + lamda0 * d1 * u1);
-  fp[2] = T(0.5) * (d1 * v1 * un + A2 * p1 + lamda0 * d1 * v1);
-  fp[3] = T(0.5) * (d1 * w1 * un + A3 * p1 + lamda0 * d1 * w1);
-  fp[4] = T(0.5) * (h0 * un + lamda0 * E0);
-  fm[0] = T(0.5) * (d1 * un - lamda0 * d1);
-  fm[1] = T(0.5) * (d1 * u1 * un + A1 * p1 - lamda0 * d1 * u1);
-  fm[2] = T(0.5) * (d1 * v1 * un + A2 * p1 - lamda0 * d1 * v1);
-  fm[3] = T(0.5) * (d1 * w1 * un + A3 * p1 - lamda0 * d1 * w1);
-  fm[4] = T(0.5) * (h0 * un - lamda0 * E0);
+  // (a*b + c*d (+ e*f)) as nested madd; x - y*z as madd(-y, z, x)
+  const T ld = lamda0 * d1, nl = -lamda0, nld = -ld;
+  const T qu = madd(A1, p1, d1 * u1 * un), qv = madd(A2, p1, d1 * v1 * un), qw = madd(A3, p1, d1 * w1 * un);
+  fp[0] = T(0.5) * madd(lamda0, d1, d1 * un);
+  fp[1] = T(0.5) * madd(ld, u1, qu);
+  fp[2] = T(0.5) * madd(ld, v1, qv);
+  fp[3] = T(0.5) * madd(ld, w1, qw);
+  fp[4] = T(0.5) * madd(lamda0, E0, h0 * un);
+  fm[0] = T(0.5) * madd(nl, d1, d1 * un);
+  fm[1] = T(0.5) * madd(nld, u1, qu);
+  fm[2] = T(0.5) * madd(nld, v1, qv);
+  fm[3] = T(0.5) * madd(nld, w1, qw);
+  fm[4] = T(0.5) * madd(nl, E0, h0 * un);
 }
 
 // Left (S) / right (S1) eigenvector matrices of the face-averaged state
@@ -574,10 +596,14 @@ template <typename T>
 __device__ inline void char_back(const CharBasisT<T> &B, const T *h0, T *hh) {
   const T c1 = B.c1;
   hh[0] = T(1.0) * h0[0] + T(0.0) * h0[1] + T(0.0) * h0[2] + T(1.0) * h0[3] + T(1.0) * h0[4];
-  hh[1] = B.u1 * h0[0] + B.l1 * h0[1] + B.m1 * h0[2] + (B.u1 - c1 * B.n1) * h0[3] + (B.u1 + c1 * B.n1) * h0[4];
-  hh[2] = B.v1 * h0[0] + B.l2 * h0[1] + B.m2 * h0[2] + (B.v1 - c1 * B.n2) * h0[3] + (B.v1 + c1 * B.n2) * h0[4];
-  hh[3] = B.w1 * h0[0] + B.l3 * h0[1] + B.m3 * h0[2] + (B.w1 - c1 * B.n3) * h0[3] + (B.w1 + c1 * B.n3) * h0[4];
-  hh[4] = B.V2 * h0[0] + B.ul * h0[1] + B.um * h0[2] + (B.H - c1 * B.un) * h0[3] + (B.H + c1 * B.un) * h0[4];
+  // row . h0 summed left to right: a0 h0 + a1 h1 + a2 h2 + (q - c1 n) h3 + (q + c1 n) h4
+  auto row = [&](T a0, T a1, T a2, T q, T nq) {
+    return madd(madd(c1, nq, q), h0[4], madd(madd(-c1, nq, q), h0[3], madd(a2, h0[2], madd(a1, h0[1], a0 * h0[0]))));
+  };
+  hh[1] = row(B.u1, B.l1, B.m1, B.u1, B.n1);
+  hh[2] = row(B.v1, B.l2, B.m2, B.v1, B.n2);
+  hh[3] = row(B.w1, B.l3, B.m3, B.w1, B.n3);
+  hh[4] = row(B.V2, B.ul, B.um, B.H, B.un);
 }
 
 // ---------------------------------------------------------------------------
@@ -591,18 +617,21 @@ template <typename F>
 __device__ inline auto dxyz0_point(F o, int s, int n, const DCoef &c, bool cd8, bool periodic) -> decltype(o(0)) {
   typedef decltype(o(0)) T;
   const T h = T(c.h), a1 = T(c.a1), a2 = T(c.a2), a3 = T(c.a3);
+  // CD6 weights (CD4-CD6 closure of the CD8 scheme two points from a side)
+  auto cd6 = [&]() { return madd(a3, o(1) - o(-1), madd(a2, o(2) - o(-2), a1 * (o(3) - o(-3)))); };
   if (!periodic) {
     if (s == 1) return (-(T(3.0) * o(0)) + T(4.0) * o(1) - o(2)) / (T(2.0) * h);
     if (s == 2) return (o(1) - o(-1)) / (T(2.0) * h);
     if (s == 3) return (T(8.0) * (o(1) - o(-1)) - (o(2) - o(-2))) / (T(12.0) * h);
-    if (cd8 && s == 4) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
+    if (cd8 && s == 4) return cd6();
     if (s == n) return (o(-2) - T(4.0) * o(-1) + T(3.0) * o(0)) / (T(2.0) * h);
     if (s == n - 1) return (o(1) - o(-1)) / (T(2.0) * h);
     if (s == n - 2) return (T(8.0) * (o(1) - o(-1)) - (o(2) - o(-2))) / (T(12.0) * h);
-    if (cd8 && s == n - 3) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
+    if (cd8 && s == n - 3) return cd6();
   }
-  if (!cd8) return a1 * (o(3) - o(-3)) + a2 * (o(2) - o(-2)) + a3 * (o(1) - o(-1));
-  return T(c.c1) * (o(1) - o(-1)) + T(c.c2) * (o(2) - o(-2)) + T(c.c3) * (o(3) - o(-3)) + T(c.c4) * (o(4) - o(-4));
+  if (!cd8) return cd6();
+  return madd(T(c.c4), o(4) - o(-4),
+              madd(T(c.c3), o(3) - o(-3), madd(T(c.c2), o(2) - o(-2), T(c.c1) * (o(1) - o(-1)))));
 }
 
 }  // namespace ocfd

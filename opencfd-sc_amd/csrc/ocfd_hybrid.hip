@@ -188,8 +188,8 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         for (int cmp = 1; cmp < 5; cmp++) {
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            xp[k] = xp[k] + sr5[cmp] * sF[fb][k * 10 + cmp];
-            xm[k] = xm[k] + sr5[cmp] * sF[fb][k * 10 + 5 + cmp];
+            xp[k] = madd(sr5[cmp], sF[fb][k * 10 + cmp], xp[k]);
+            xm[k] = madd(sr5[cmp], sF[fb][k * 10 + 5 + cmp], xm[k]);
           }
         }
 #pragma unroll

@@ -232,8 +232,8 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        xp[k] = xp[k] + sr5[c] * FPV(c, k);
-        xm[k] = xm[k] + sr5[c] * FMV(c, k);
+        xp[k] = madd(sr5[c], FPV(c, k), xp[k]);
+        xm[k] = madd(sr5[c], FMV(c, k), xm[k]);
       }
     }
 #pragma unroll
