@@ -183,12 +183,19 @@ int set_params(ocfd_handle *h) {
   return 0;
 }
 
-__global__ void k_fill(double *a, int64_t n, double v) {
+template <typename E>
+__global__ void k_fill(E *a, int64_t n, E v) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = v;
 }
 hipError_t fill(double *a, int64_t n, double v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, v);
+  hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, v);
+  return hipGetLastError();
+}
+hipError_t fill(const PrimField &a, int64_t n, double v, hipStream_t st) {
+  if (!a.f32) return fill(a.as<double>(), n, v, st);
+  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a.as<float>(), n,
+                     float(v));
   return hipGetLastError();
 }
 
@@ -221,10 +228,80 @@ int alloc(double **p, size_t bytes) {
   }
   return 0;
 }
+// a halo'd primitive field of nh points: FP32 in the mixed-precision form
+int alloc_prim(PrimField *p, int64_t nh, bool f32) {
+  p->f32 = f32 ? 1 : 0;
+  const size_t bytes = (size_t)nh * (f32 ? sizeof(float) : sizeof(double));
+  if (hipMalloc(&p->p, bytes) != hipSuccess) {
+    p->p = nullptr;
+    return fail("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return 0;
+}
 
-double *prims_of(ocfd_handle *h, int q) {
-  double *p[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
+PrimField prims_of(ocfd_handle *h, int q) {
+  const PrimField p[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
   return p[q];
+}
+
+// primitive q, halo'd, to / from nh doubles on the host (converted from and
+// to FP32 in the mixed-precision form)
+int prim_download(ocfd_handle *h, int q, double *dst) {
+  const PrimField p = prims_of(h, q);
+  const size_t n = (size_t)h->B.g.nh;
+  if (!p.f32) {
+    HIPCHK(hipMemcpy(dst, p.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<float> tmp(n);
+  HIPCHK(hipMemcpy(tmp.data(), p.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) dst[i] = (double)tmp[i];
+  return 0;
+}
+int prim_upload(ocfd_handle *h, int q, const double *src) {
+  const PrimField p = prims_of(h, q);
+  const size_t n = (size_t)h->B.g.nh;
+  if (!p.f32) {
+    HIPCHK(hipMemcpy(p.p, src, n * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  }
+  std::vector<float> tmp(n);
+  for (size_t i = 0; i < n; i++) tmp[i] = (float)src[i];
+  HIPCHK(hipMemcpy(p.p, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// the primitives from nh doubles per field on the host; with make_f, f from
+// those FP64 values as at start-up (init3d, OCFD_init.f90:10-20).  In the
+// mixed-precision form the FP64 values pass through device scratch fields, so
+// that the conserved variables are formed from them and not from their FP32
+// rounding.
+int upload_prims(ocfd_handle *h, const double *const src[5], bool make_f) {
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (!h->B.d.f32) {
+    for (int q = 0; q < 5; q++)
+      if (prim_upload(h, q, src[q])) return -1;
+    if (make_f && launch(h, "init_f", [&] { return launch_init_f(h->B, h->st); })) return -1;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+  }
+  int rc = 0;
+  if (make_f) {
+    Block B64 = h->B;
+    PrimField *f64[5] = {&B64.d, &B64.u, &B64.v, &B64.w, &B64.T};
+    for (int q = 0; q < 5; q++) f64[q]->p = nullptr;
+    for (int q = 0; q < 5 && !rc; q++) {
+      rc = alloc_prim(f64[q], B64.g.nh, false);
+      if (!rc && hipMemcpy(f64[q]->p, src[q], h->halo_bytes(), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail("copy of the primitives failed");
+    }
+    if (!rc && launch(h, "init_f", [&] { return launch_init_f(B64, h->st); })) rc = -1;
+    if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail("init_f failed");
+    for (int q = 0; q < 5; q++)
+      if (f64[q]->p) (void)hipFree(f64[q]->p);
+  }
+  for (int q = 0; q < 5 && !rc; q++) rc = prim_upload(h, q, src[q]);
+  return rc;
 }
 
 int inviscid(ocfd_handle *h, bool ch) {
@@ -247,8 +324,8 @@ int inviscid(ocfd_handle *h, bool ch) {
 }
 
 int exchange_prims(ocfd_handle *h) {
-  double *f[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
-  return launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, f, 5, h->st); });
+  const PrimField f[5] = {h->B.d, h->B.u, h->B.v, h->B.w, h->B.T};
+  return launch(h, "halo_fill", [&] { return launch_halo_prims(h->B, f, 5, h->st); });
 }
 
 int sensor(ocfd_handle *h) {
@@ -342,7 +419,10 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   Block &B = h->B;
   const size_t hb = h->halo_bytes(), fb = h->f_bytes();
   int rc = 0;
-  rc |= alloc(&B.d, hb) | alloc(&B.u, hb) | alloc(&B.v, hb) | alloc(&B.w, hb) | alloc(&B.T, hb);
+  // the primitives are FP32 in the mixed-precision form
+  const bool p32 = B.prec == OCFD_PREC_MIXED_CODE;
+  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) p->p = nullptr;
+  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) rc |= alloc_prim(p, B.g.nh, p32);
   rc |= alloc(&B.f, fb) | alloc(&B.fn, fb) | alloc(&B.du, fb);
   if (cfg->scheme_invis == SCH_HYBRID) rc |= alloc(&B.rhyb, hb);
   if (cfg->if_viscous == 1) rc |= alloc(&B.ev, 12 * (size_t)B.g.ni * sizeof(double));
@@ -382,8 +462,7 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
     B.M.u[m] = 1.0;
   }
   // allocate_flow_data initial values (OCFD_NS_Solver.f90:116-118, 140-147)
-  double *hal[5] = {B.d, B.u, B.v, B.w, B.T};
-  for (int q = 0; q < 5; q++) (void)fill(hal[q], B.g.nh, 1.0, h->st);
+  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) (void)fill(*p, B.g.nh, 1.0, h->st);
   if (!B.M.uniform)
     for (int m = 0; m < M_N; m++) (void)fill(h->met[m], B.g.nh, 1.0, h->st);
   if (B.rhyb) (void)fill(B.rhyb, B.g.nh, 0.0, h->st);
@@ -416,7 +495,9 @@ int ocfd_destroy(ocfd_handle *h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   invalidate_graph(h);
   Block &B = h->B;
-  double *ps[] = {B.d, B.u, B.v, B.w, B.T, B.f, B.fn, B.du, B.rhyb, B.ev, B.trash, h->fbuf, h->work, h->wout};
+  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T})
+    if (p->p) (void)hipFree(p->p);
+  double *ps[] = {B.f, B.fn, B.du, B.rhyb, B.ev, B.trash, h->fbuf, h->work, h->wout};
   for (double *p : ps)
     if (p) (void)hipFree(p);
   for (int a = 0; a < 3; a++) {
@@ -499,12 +580,10 @@ int ocfd_download_metrics(ocfd_handle *h, double *const metrics[19]) {
 // ---- state -----------------------------------------------------------------
 int ocfd_upload_primitives(ocfd_handle *h, const double *d, const double *u, const double *v, const double *w,
                            const double *T, int32_t istep, double tt) {
-  const double *src[5] = {d, u, v, w, T};
-  for (int q = 0; q < 5; q++) HIPCHK(hipMemcpy(prims_of(h, q), src[q], h->halo_bytes(), hipMemcpyHostToDevice));
+  const double *const src[5] = {d, u, v, w, T};
+  if (upload_prims(h, src, true)) return -1;
   h->istep = istep;
   h->tt = tt;
-  if (launch(h, "init_f", [&] { return launch_init_f(h->B, h->st); })) return -1;
-  HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }
 
@@ -552,7 +631,7 @@ int ocfd_save_flow_data(ocfd_handle *h, const char *path, int32_t with_f) {
   // k plane of the nx x ny interior
   std::vector<double> buf((size_t)g.nh), plane((size_t)g.nx * g.ny);
   for (int q = 0; q < 5; q++) {
-    HIPCHK(hipMemcpy(buf.data(), prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+    if (prim_download(h, q, buf.data())) return -1;
     for (int k = 1; k <= g.nz; k++) {
       for (int j = 1; j <= g.ny; j++)
         std::memcpy(&plane[(size_t)(j - 1) * g.nx], &buf[(size_t)g.hi(1, j, k)], (size_t)g.nx * sizeof(double));
@@ -585,9 +664,11 @@ int ocfd_read_flow_data(ocfd_handle *h, const char *path, int32_t *has_f) {
   // read_flow_data (OCFD_IO.f90:5-39) / read_3d1: only the interior of each
   // primitive is replaced; the halos keep their values
   HIPCHK(hipStreamSynchronize(h->st));
-  std::vector<double> buf((size_t)g.nh), plane((size_t)g.nx * g.ny);
+  std::vector<std::vector<double>> bufs(5, std::vector<double>((size_t)g.nh));
+  std::vector<double> plane((size_t)g.nx * g.ny);
   for (int q = 0; q < 5; q++) {
-    HIPCHK(hipMemcpy(buf.data(), prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+    std::vector<double> &buf = bufs[q];
+    if (prim_download(h, q, buf.data())) return -1;
     for (int k = 1; k <= g.nz; k++) {
       if (read_rec(fp, plane.data(), (int64_t)plane.size() * 8) != 1)
         return fail("malformed flow-data file (field " + std::to_string(q + 1) + ", plane " + std::to_string(k) +
@@ -595,7 +676,6 @@ int ocfd_read_flow_data(ocfd_handle *h, const char *path, int32_t *has_f) {
       for (int j = 1; j <= g.ny; j++)
         std::memcpy(&buf[(size_t)g.hi(1, j, k)], &plane[(size_t)(j - 1) * g.nx], (size_t)g.nx * sizeof(double));
     }
-    HIPCHK(hipMemcpy(prims_of(h, q), buf.data(), h->halo_bytes(), hipMemcpyHostToDevice));
   }
   h->istep = istep;
   h->tt = tt;
@@ -604,14 +684,13 @@ int ocfd_read_flow_data(ocfd_handle *h, const char *path, int32_t *has_f) {
   std::vector<double> f((size_t)5 * g.ni);
   int rc = read_rec(fp, f.data(), (int64_t)g.nx * g.ny * 8);
   if (rc < 0) return fail("malformed conserved-variable records");
-  if (rc == 1) {
+  if (rc == 1)
     for (int64_t r = 1; r < 5 * (int64_t)g.nz; r++)
       if (read_rec(fp, &f[(size_t)(r * g.nx * g.ny)], (int64_t)g.nx * g.ny * 8) != 1)
         return fail("malformed conserved-variable records");
-    HIPCHK(hipMemcpy(h->B.f, f.data(), h->f_bytes(), hipMemcpyHostToDevice));
-  } else if (launch(h, "init_f", [&] { return launch_init_f(h->B, h->st); })) {
-    return -1;
-  }
+  const double *const src[5] = {bufs[0].data(), bufs[1].data(), bufs[2].data(), bufs[3].data(), bufs[4].data()};
+  if (upload_prims(h, src, rc != 1)) return -1;
+  if (rc == 1) HIPCHK(hipMemcpy(h->B.f, f.data(), h->f_bytes(), hipMemcpyHostToDevice));
   HIPCHK(hipStreamSynchronize(h->st));
   if (has_f) *has_f = rc == 1 ? 1 : 0;
   return 0;
@@ -660,7 +739,7 @@ int ocfd_download_primitives(ocfd_handle *h, double *d, double *u, double *v, do
   HIPCHK(hipStreamSynchronize(h->st));
   double *dst[5] = {d, u, v, w, T};
   for (int q = 0; q < 5; q++)
-    if (dst[q]) HIPCHK(hipMemcpy(dst[q], prims_of(h, q), h->halo_bytes(), hipMemcpyDeviceToHost));
+    if (dst[q] && prim_download(h, q, dst[q])) return -1;
   return 0;
 }
 int ocfd_download_rhybrid(ocfd_handle *h, double *r) {
@@ -965,8 +1044,8 @@ int ocfd_show_flow_msg(ocfd_handle *h, double out[3]) {
 }
 
 int ocfd_ana_box(ocfd_handle *h, double out[20]) {
-  double *uvw[3] = {h->B.u, h->B.v, h->B.w};
-  if (launch(h, "halo_fill", [&] { return launch_halo_fill(h->B, uvw, 3, h->st); })) return -1;
+  const PrimField uvw[3] = {h->B.u, h->B.v, h->B.w};
+  if (launch(h, "halo_fill", [&] { return launch_halo_prims(h->B, uvw, 3, h->st); })) return -1;
   if (launch(h, "ana_box", [&] { return launch_ana_box(h->B, nullptr, nullptr, h->work, h->wout, h->st); }))
     return -1;
   double s[15];

@@ -9,7 +9,7 @@ namespace ocfd {
 hipError_t launch_inviscid_mixed(const Block &B, int axis, int mode, int shape, hipStream_t st);
 
 hipError_t launch_inviscid(const Block &B, int axis, int mode, SweepTile tile, hipStream_t st) {
-  if (tile.shape < 0 || tile.shape >= NSWEEP_SHAPES) return hipErrorInvalidValue;
+  if (tile.shape < 0 || tile.shape >= NSWEEP_SHAPES || !prims_match(B)) return hipErrorInvalidValue;
   if (B.prec == OCFD_PREC_MIXED_CODE) return launch_inviscid_mixed(B, axis, mode, tile.shape, st);
   return launch_sweep<double>(B, axis, mode, tile.shape, st);
 }

@@ -5,13 +5,40 @@
 
 namespace ocfd {
 
+// A halo'd primitive field (d, u, v, w or T): FP64, or FP32 in the
+// mixed-precision form, where the primitives are derived values read only by
+// the FP32 kernels (the conserved variables f stay FP64).  The sweeps and
+// the viscous kernels take the typed pointer, as<R>() with R their real
+// type; other code indexes the field and reads and writes doubles.
+struct PrimField {
+  void *p;
+  int f32;
+  template <typename R>
+  __host__ __device__ R *as() const { return static_cast<R *>(p); }
+  struct Ref {
+    void *p;
+    int f32;
+    int64_t i;
+    __device__ operator double() const {
+      return f32 ? double(static_cast<const float *>(p)[i]) : static_cast<const double *>(p)[i];
+    }
+    __device__ Ref &operator=(double x) {
+      if (f32) static_cast<float *>(p)[i] = float(x);
+      else static_cast<double *>(p)[i] = x;
+      return *this;
+    }
+    __device__ Ref &operator=(const Ref &r) { return *this = double(r); }
+  };
+  __device__ Ref operator[](int64_t i) const { return Ref{p, f32, i}; }
+};
+
 // Everything a kernel may need about the block; passed by value.
 struct Block {
   Geom g;
   SchemeParams S;
   Phys P;
   Metrics M;
-  double *d, *u, *v, *w, *T;  // halo'd primitives
+  PrimField d, u, v, w, T;    // halo'd primitives
   double *f, *fn, *du;        // 5 planes each
   double *rhyb;               // halo'd sensor (hybrid scheme only)
   double *ev;                 // 12 interior planes: Ev1(1:4), Ev2(1:4), Ev3(1:4)
@@ -24,6 +51,12 @@ struct Block {
   int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
   int kseg;  // boxes of a batch: planes per workgroup of the k-marching kernels
 };
+// the kernels of a precision read the primitives in their real type: FP32
+// fields belong to the mixed-precision form, FP64 fields to the FP64 path
+inline bool prims_match(const Block &B) {
+  const int f32 = B.prec == OCFD_PREC_MIXED_CODE;
+  return B.d.f32 == f32 && B.u.f32 == f32 && B.v.f32 == f32 && B.w.f32 == f32 && B.T.f32 == f32;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2.  xcd_tile
@@ -163,6 +196,8 @@ constexpr int BATCH_KSEGS[NBATCH_KSEG] = {4, 8, 16, 32};
 int64_t halo_tiles(const Geom &g);
 hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st);
 hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st);
+// the same for primitive fields (at most 5, all of one precision)
+hipError_t launch_halo_prims(const Block &B, const PrimField *fields, int nfields, hipStream_t st);
 hipError_t launch_sensor(const Block &B, hipStream_t st);
 hipError_t launch_sensor2(const Block &B, double *work, hipStream_t st);
 hipError_t launch_hybrid_counts(const Block &B, unsigned long long *counts, hipStream_t st);

@@ -29,9 +29,11 @@ __device__ inline void cell_of(const Geom &g, int64_t t, int &i, int &j, int &k)
 // indices (the extent of TYPE_LAP*2, OCFD_mpi.f90:114-122).
 // ---------------------------------------------------------------------------
 struct HaloFields {
-  double *p[8];
+  void *p[8];
+  int f32;  // float fields (primitives of the mixed-precision form)
 };
-__device__ inline void halo_point(const Geom &g, double *const *P, const int nf, const int px, const int py,
+template <typename E>
+__device__ inline void halo_point(const Geom &g, void *const *P, const int nf, const int px, const int py,
                                   const int pz, const int64_t t) {
   const int64_t nxh = px ? (int64_t)2 * LAP * g.ny * g.nz : 0;
   const int64_t nyh = py ? (int64_t)2 * LAP * g.nx * g.nz : 0;
@@ -60,33 +62,40 @@ __device__ inline void halo_point(const Geom &g, double *const *P, const int nf,
     if (l < LAP) { dst = g.hi(i, j, g.nz + 1 + l); src = g.hi(i, j, 1 + l); }
     else { const int q = l - LAP; dst = g.hi(i, j, 1 - LAP + q); src = g.hi(i, j, g.nz - LAP + 1 + q); }
   }
-  for (int q = 0; q < nf; q++) P[q][dst] = P[q][src];
+  for (int q = 0; q < nf; q++) static_cast<E *>(P[q])[dst] = static_cast<E *>(P[q])[src];
 }
 
 __global__ void k_halo(const Geom g, const HaloFields F, const int nf, const int px, const int py, const int pz) {
-  halo_point(g, F.p, nf, px, py, pz, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (F.f32) halo_point<float>(g, F.p, nf, px, py, pz, t);
+  else halo_point<double>(g, F.p, nf, px, py, pz, t);
 }
 
 // the five primitives of every box of a batch (all axes periodic)
 __global__ __launch_bounds__(NT) void k_halo_batch(const BoxSet S) {
   const int j = S.entry(blockIdx.x);
   const Block &B = S.blk[S.ids[j]];
-  double *const P[5] = {B.d, B.u, B.v, B.w, B.T};
-  halo_point(B.g, P, 5, 1, 1, 1, (int64_t)(blockIdx.x - S.off[j]) * NT + threadIdx.x);
+  void *const P[5] = {B.d.p, B.u.p, B.v.p, B.w.p, B.T.p};
+  const int64_t t = (int64_t)(blockIdx.x - S.off[j]) * NT + threadIdx.x;
+  if (B.d.f32) halo_point<float>(B.g, P, 5, 1, 1, 1, t);
+  else halo_point<double>(B.g, P, 5, 1, 1, 1, t);
 }
 
 // ---------------------------------------------------------------------------
 // shock sensor 1 (OCFD_SetHybrid.f90:240-257): R = sum over axes of
 // |p(-1) - 2p + p(+1)| / |p(-1) + 2p + p(+1)|, p = p00 d T
 // ---------------------------------------------------------------------------
+// E: element type of the primitive fields
+template <typename E>
 __global__ __launch_bounds__(NT) void k_sensor1(const Block B, const double p00) {
   const Geom &g = B.g;
+  const E *const pd = B.d.as<E>(), *const pT = B.T.as<E>();
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= g.ni) return;
   int i, j, k;
   cell_of(g, t, i, j, k);
   const int64_t h = g.hi(i, j, k);
-  auto P = [&](int64_t q) { return p00 * B.d[q] * B.T[q]; };
+  auto P = [&](int64_t q) { return p00 * double(pd[q]) * double(pT[q]); };
   const double pc = P(h);
   double r = 0.0;
   double acc[3];
@@ -387,7 +396,7 @@ __global__ __launch_bounds__(NT) void k_bl_outlet(const Block B, const BLArgs a)
   if (t >= (int64_t)g.ny * g.nz) return;
   const int j = (int)(t % g.ny) + 1, k = (int)(t / g.ny) + 1;
   const int64_t h = g.hi(g.nx, j, k), h1 = g.hi(g.nx - 1, j, k), h2 = g.hi(g.nx - 2, j, k);
-  double *F[5] = {B.d, B.u, B.v, B.w, B.T};
+  const PrimField F[5] = {B.d, B.u, B.v, B.w, B.T};
   for (int q = 0; q < 5; q++) F[q][h] = a.outlet == 1 ? F[q][h1] : 2.0 * F[q][h1] - F[q][h2];
 }
 
@@ -397,7 +406,7 @@ __global__ __launch_bounds__(NT) void k_bl_sides(const Block B, const BLArgs a) 
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.nx * g.ny) return;
   const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
-  double *F[5] = {B.d, B.u, B.v, B.w, B.T};
+  const PrimField F[5] = {B.d, B.u, B.v, B.w, B.T};
   if (a.z1 == 1)
     for (int q = 0; q < 5; q++) F[q][g.hi(i, j, 1)] = F[q][g.hi(i, j, 2)];
   if (a.z2 == 1)
@@ -618,13 +627,13 @@ __global__ __launch_bounds__(NT) void k_ana_box(const Block B, double *partial, 
       s[0] = x * x;
     } else {
       double gr[9];
-      const double *fld[3] = {B.u, B.v, B.w};
+      const PrimField fld[3] = {B.u, B.v, B.w};
       for (int c = 0; c < 3; c++)
         for (int a = 0; a < 3; a++) {
           const int64_t st = g.hstride(a);
           const int sidx = a == 0 ? i : (a == 1 ? j : k);
-          const double *arr = fld[c];
-          auto at = [&](int q) { return arr[h + (int64_t)q * st]; };
+          const PrimField arr = fld[c];
+          auto at = [&](int q) { return double(arr[h + (int64_t)q * st]); };
           gr[c * 3 + a] =
               dxyz0_point(at, sidx, g.n(a), B.P.dc[a], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[a] == 1);
         }
@@ -852,7 +861,8 @@ __global__ __launch_bounds__(NT) void k_normalise(const Geom g, const MetPtrs mp
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st) {
+namespace {
+hipError_t halo_fill(const Block &B, void *const *fields, int nfields, int f32, hipStream_t st) {
   const Geom &g = B.g;
   const int px = B.S.iperiodic[0] == 1, py = B.S.iperiodic[1] == 1, pz = B.S.iperiodic[2] == 1;
   const int64_t tot = (px ? (int64_t)2 * LAP * g.ny * g.nz : 0) + (py ? (int64_t)2 * LAP * g.nx * g.nz : 0) +
@@ -862,14 +872,34 @@ hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, 
     HaloFields F;
     const int nf = nfields - f0 < 8 ? nfields - f0 : 8;
     for (int q = 0; q < nf; q++) F.p[q] = fields[f0 + q];
+    F.f32 = f32;
     hipLaunchKernelGGL(k_halo, dim3(nb(tot)), dim3(NT), 0, st, g, F, nf, px, py, pz);
   }
   return hipGetLastError();
 }
+}  // namespace
+
+hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st) {
+  void *p[M_N + 8];
+  if (nfields > M_N + 8) return hipErrorInvalidValue;
+  for (int q = 0; q < nfields; q++) p[q] = fields[q];
+  return halo_fill(B, p, nfields, 0, st);
+}
+
+hipError_t launch_halo_prims(const Block &B, const PrimField *fields, int nfields, hipStream_t st) {
+  void *p[5];
+  if (nfields > 5) return hipErrorInvalidValue;
+  for (int q = 0; q < nfields; q++) {
+    if (fields[q].f32 != fields[0].f32) return hipErrorInvalidValue;
+    p[q] = fields[q].p;
+  }
+  return halo_fill(B, p, nfields, fields[0].f32, st);
+}
 
 hipError_t launch_sensor(const Block &B, hipStream_t st) {
   const double p00 = 1.0 / (B.P.gamma * B.P.ma * B.P.ma);
-  hipLaunchKernelGGL(k_sensor1, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, p00);
+  if (B.d.f32) hipLaunchKernelGGL(k_sensor1<float>, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, p00);
+  else hipLaunchKernelGGL(k_sensor1<double>, dim3(nb(B.g.ni)), dim3(NT), 0, st, B, p00);
   return hipGetLastError();
 }
 

@@ -82,19 +82,21 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     return it < 5 * TL * TW && s0 + sl_ <= n && w0 + wl <= nw;
   };
 
-  // primitives (+ sensor) of the tile's points
-  double pq[NI1][5];
+  // primitives of the tile's points (stored in the sweep's real type)
+  R pq[NI1][5];
   {
+    const R *const pd = B.d.as<R>(), *const pu = B.u.as<R>(), *const pv = B.v.as<R>(), *const pw = B.w.as<R>(),
+            *const pT = B.T.as<R>();
 #pragma unroll
     for (int r = 0; r < NI1; r++) {
       int p, wl;
       if (!p1(r, p, wl)) continue;
       const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
-      pq[r][0] = B.d[h];
-      pq[r][1] = B.u[h];
-      pq[r][2] = B.v[h];
-      pq[r][3] = B.w[h];
-      pq[r][4] = B.T[h];
+      pq[r][0] = pd[h];
+      pq[r][1] = pu[h];
+      pq[r][2] = pv[h];
+      pq[r][3] = pw[h];
+      pq[r][4] = pT[h];
     }
   }
   // du entries of the tile (accumulating modes), fetched before the face

@@ -116,7 +116,8 @@ __device__ __forceinline__ void visc_flux_tile(const Block &B, const int kseg, c
   __shared__ RT sT[2][4 * NPL];  // u, v, w, T of one plane: [4][LY][LX]
   const Geom &g = B.g;
   const TileCols tc = tile_cols(g, kseg, tile);
-  const double *F[4] = {B.u, B.v, B.w, B.T};
+  // u, v, w, T in the kernel's real type (FP32 fields in the mixed form)
+  const RT *F[4] = {B.u.as<RT>(), B.v.as<RT>(), B.w.as<RT>(), B.T.as<RT>()};
   const int64_t sk = g.hstride(2);
   const int ic = tc.i < g.nx ? tc.i : g.nx, jc = tc.j < g.ny ? tc.j : g.ny;
 
@@ -473,13 +474,14 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
     double d, u, v, w, T;
     prims_of(B.P.cv, fq, d, u, v, w, T);
     {
+      // the primitives are stored in the kernel's real type
       const int64_t ph = tc.valid ? h : 0;
-      double *pt = B.trash + 10 * NT + threadIdx.x;
-      (tc.valid ? B.d + ph : pt)[0] = d;
-      (tc.valid ? B.u + ph : pt + NT)[0] = u;
-      (tc.valid ? B.v + ph : pt + 2 * NT)[0] = v;
-      (tc.valid ? B.w + ph : pt + 3 * NT)[0] = w;
-      (tc.valid ? B.T + ph : pt + 4 * NT)[0] = T;
+      RT *pt = reinterpret_cast<RT *>(B.trash + 10 * NT) + threadIdx.x;
+      (tc.valid ? B.d.as<RT>() + ph : pt)[0] = RT(d);
+      (tc.valid ? B.u.as<RT>() + ph : pt + NT)[0] = RT(u);
+      (tc.valid ? B.v.as<RT>() + ph : pt + 2 * NT)[0] = RT(v);
+      (tc.valid ? B.w.as<RT>() + ph : pt + 3 * NT)[0] = RT(w);
+      (tc.valid ? B.T.as<RT>() + ph : pt + 4 * NT)[0] = RT(T);
     }
     negmask |= (tc.valid && T <= 0) ? (1u << (k - tc.kb)) : 0u;
     if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz))

@@ -14,6 +14,7 @@ hipError_t launch_visc_div_update_batch_mixed(const BoxSet &S, bool visc, bool c
                                               hipStream_t st);
 
 hipError_t launch_visc_flux(const Block &B, hipStream_t st) {
+  if (!prims_match(B)) return hipErrorInvalidValue;
   if (B.prec == OCFD_PREC_MIXED_CODE) return launch_visc_flux_mixed(B, st);
   return visc_flux_as<double>(B, st);
 }
@@ -24,6 +25,7 @@ hipError_t launch_visc_div(const Block &B, hipStream_t st) {
 }
 
 hipError_t launch_visc_div_update(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
+  if (!prims_match(B)) return hipErrorInvalidValue;
   if (B.prec == OCFD_PREC_MIXED_CODE) return launch_visc_div_update_mixed(B, krk, refresh_planes, neg, st);
   return visc_div_update_as<double>(B, krk, refresh_planes, neg, st);
 }
