@@ -44,10 +44,13 @@ enum { OCFD_SPLIT_SW = 1, OCFD_SPLIT_LLF = 2 };
 enum { OCFD_BC_NONE = 0, OCFD_BC_BLUNT2D = 1, OCFD_BC_BOUNDARYLAYER = 2, OCFD_BC_SWEPTCORNER = 3, OCFD_BC_USER_DEF = 99 };
 /* Arithmetic of the right-hand side.  OCFD_PREC_MIXED forms the flux
    splitting, the reconstructions of the sweeps and the viscous derivatives,
-   stresses and fluxes and accumulates du in FP32; the primitives, f, fn, the
-   RK3 stage combination, metrics and Jacobian stay FP64, and so do the
-   characteristic projection and WENO reconstruction of the faces the hybrid
-   sensor flags.  ocfd_download_du / ocfd_upload_du convert du on the host. */
+   stresses and fluxes and accumulates du in FP32, and stores the primitives
+   d, u, v, w, T (derived values, recomputed from f every stage in FP64) as
+   FP32; the conserved variables f, fn, the RK3 stage combination, metrics and
+   Jacobian stay FP64, and so do the characteristic projection and WENO
+   reconstruction of the faces the hybrid sensor flags.  The transfers of du
+   and of the primitives (upload, download, flow-data files) convert on the
+   host; an upload of the primitives forms f from the FP64 values given. */
 enum { OCFD_PREC_FP64 = 0, OCFD_PREC_MIXED = 1 };
 /* How grid metrics are held on the device. */
 enum { OCFD_METRIC_FIELD = 0, OCFD_METRIC_UNIFORM = 1 };
