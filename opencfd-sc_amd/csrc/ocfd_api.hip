@@ -78,8 +78,11 @@ hipEvent_t ev_get(ocfd_handle *h) {
     h->pool.pop_back();
     return e;
   }
+  // timing only: no system-scope fence (cache writeback) around the kernel,
+  // which would add to the time measured; flush_prof synchronizes the stream
+  // before reading the events
   hipEvent_t e;
-  (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 

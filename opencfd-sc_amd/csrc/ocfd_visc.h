@@ -230,15 +230,17 @@ __device__ __forceinline__ void visc_flux_tile(const Block &B, const int kseg, c
     const RT E1 = uu * s11 + vv * s12 + ww * s13 + Amuk * Tx;
     const RT E2 = uu * s12 + vv * s22 + ww * s23 + Amuk * Ty;
     const RT E3 = uu * s13 + vv * s23 + ww * s33 + Amuk * Tz;
+    // Ev is stored non-temporally: the L2 keeps the primitive planes that
+    // the neighbouring tiles read as their halos
     RT *eb = tc.valid ? ev + g.ii(i, j, k) : trash + threadIdx.x;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       const RT n1 = met(M_AKX1 + 3 * a, h), n2 = met(M_AKX1 + 3 * a + 1, h), n3 = met(M_AKX1 + 3 * a + 2, h);
       RT *e = eb + (int64_t)(4 * a) * es;
-      e[0] = n1 * s11 + n2 * s12 + n3 * s13;
-      e[es] = n1 * s12 + n2 * s22 + n3 * s23;
-      e[2 * es] = n1 * s13 + n2 * s23 + n3 * s33;
-      e[3 * es] = n1 * E1 + n2 * E2 + n3 * E3;
+      __builtin_nontemporal_store(n1 * s11 + n2 * s12 + n3 * s13, &e[0]);
+      __builtin_nontemporal_store(n1 * s12 + n2 * s22 + n3 * s23, &e[es]);
+      __builtin_nontemporal_store(n1 * s13 + n2 * s23 + n3 * s33, &e[2 * es]);
+      __builtin_nontemporal_store(n1 * E1 + n2 * E2 + n3 * E3, &e[3 * es]);
     }
     // plane k+1 into the other buffer (its last readers, plane k-1, are past
     // the previous barrier); the queue moves up one plane
@@ -404,7 +406,10 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
   const int i = ic, j = jc;
   const int64_t os = tc.valid ? ni : NT;  // output plane stride (trash for idle lanes)
   // du, f, fn of the lane's cell at plane k, fetched one plane ahead (a lane
-  // writes f, fn only at its own cell of the current plane)
+  // writes f, fn only at its own cell of the current plane).  Data read or
+  // written once per stage -- du, f, fn, the primitives -- moves with
+  // non-temporal loads and stores, which leaves the L2 to the Ev tiles the
+  // neighbouring tiles share.
   struct CellVals {
     double du[5], f[5], fn[5];
   };
@@ -413,9 +418,9 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
     const int64_t t = col + (int64_t)(k - 1) * skz;
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-      c.du[m] = double(dus[t + m * ni]);
-      c.f[m] = B.f[t + m * ni];
-      c.fn[m] = krk != 1 ? B.fn[t + m * ni] : 0.0;
+      c.du[m] = double(__builtin_nontemporal_load(&dus[t + m * ni]));
+      c.f[m] = __builtin_nontemporal_load(&B.f[t + m * ni]);
+      c.fn[m] = krk != 1 ? __builtin_nontemporal_load(&B.fn[t + m * ni]) : 0.0;
     }
     return c;
   };
@@ -462,7 +467,7 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
       const double f = f0[m];
       double r;
       if (krk == 1) {
-        fno[m * os] = f;
+        __builtin_nontemporal_store(f, &fno[m * os]);
         r = f + dt * dum;
       } else {
         const double ra = krk == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
@@ -477,17 +482,17 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
       // the primitives are stored in the kernel's real type
       const int64_t ph = tc.valid ? h : 0;
       RT *pt = reinterpret_cast<RT *>(B.trash + 10 * NT) + threadIdx.x;
-      (tc.valid ? B.d.as<RT>() + ph : pt)[0] = RT(d);
-      (tc.valid ? B.u.as<RT>() + ph : pt + NT)[0] = RT(u);
-      (tc.valid ? B.v.as<RT>() + ph : pt + 2 * NT)[0] = RT(v);
-      (tc.valid ? B.w.as<RT>() + ph : pt + 3 * NT)[0] = RT(w);
-      (tc.valid ? B.T.as<RT>() + ph : pt + 4 * NT)[0] = RT(T);
+      __builtin_nontemporal_store(RT(d), tc.valid ? B.d.as<RT>() + ph : pt);
+      __builtin_nontemporal_store(RT(u), tc.valid ? B.u.as<RT>() + ph : pt + NT);
+      __builtin_nontemporal_store(RT(v), tc.valid ? B.v.as<RT>() + ph : pt + 2 * NT);
+      __builtin_nontemporal_store(RT(w), tc.valid ? B.w.as<RT>() + ph : pt + 3 * NT);
+      __builtin_nontemporal_store(RT(T), tc.valid ? B.T.as<RT>() + ph : pt + 4 * NT);
     }
     negmask |= (tc.valid && T <= 0) ? (1u << (k - tc.kb)) : 0u;
     if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz))
       refresh_of(B.P.cv, d, u, v, w, T, fq);
 #pragma unroll
-    for (int m = 0; m < 5; m++) fo[m * os] = fq[m];
+    for (int m = 0; m < 5; m++) __builtin_nontemporal_store(fq[m], &fo[m * os]);
     cv = cn;
     if (VISC) {
       store_tile(sE[(k + 1 - tc.kb) & 1]);
