@@ -485,26 +485,56 @@ __device__ inline T face_m_char(const SchemeParams &S, const T *x, int s, int n,
 // ---------------------------------------------------------------------------
 // flux-vector splitting of one point (OCFD_flux_split.f90)
 // ---------------------------------------------------------------------------
+// Constants of the splittings at a point: functions of gamma and of the
+// metric vector A only.  A sweep forms the gamma part once per thread and,
+// on a uniform grid, the metric part too; the operations are the routines'
+// own, so the fluxes are unchanged bit for bit.
+template <typename T>
+struct SplitK {
+  T gamma, gm1, g2;              // gamma, gamma - 1, 2 gamma
+  T sw1, sw3;                    // split_SW: 2 (gamma - 1), (3 - gamma) / (2 (gamma - 1))
+  T llf0, llf1;                  // split_LLF: 1 / gamma, 1 / (gamma - 1)
+  T A1, A2, A3, ss, ss1, ak1, ak2, ak3;  // A, |A|, and for split_SW 1/|A|, A/|A|
+};
+template <typename T>
+__device__ inline void split_k_gamma(T gamma, SplitK<T> &K) {
+  K.gamma = gamma;
+  K.gm1 = gamma - T(1.0);
+  K.g2 = T(2.0) * gamma;
+  K.sw1 = T(2.0) * (gamma - T(1.0));
+  K.sw3 = (T(3.0) - gamma) / (T(2.0) * (gamma - T(1.0)));
+  K.llf0 = T(1.0) / gamma;
+  K.llf1 = T(1.0) / (gamma - T(1.0));
+}
+template <typename T>
+__device__ inline void split_k_metric(T A1, T A2, T A3, bool sw, SplitK<T> &K) {
+  K.A1 = A1;
+  K.A2 = A2;
+  K.A3 = A3;
+  K.ss = sqrt(madd(A3, A3, madd(A2, A2, A1 * A1)));
+  if (sw) {
+    K.ss1 = T(1.0) / K.ss;
+    K.ak1 = A1 * K.ss1;
+    K.ak2 = A2 * K.ss1;
+    K.ak3 = A3 * K.ss1;
+  }
+}
+
 // split_Stager_Warming (OCFD_flux_split.f90:5-70)
 template <typename T>
-__device__ inline void split_sw(T gamma, T c1, T d1, T u1, T v1, T w1, T A1,
-                                T A2, T A3, T *fp, T *fm) {
-  const T tmp1 = T(2.0) * (gamma - T(1.0));
-  const T tmp3 = (T(3.0) - gamma) / (T(2.0) * (gamma - T(1.0)));
-  T ss = sqrt(madd(A3, A3, madd(A2, A2, A1 * A1)));
-  T ss1 = T(1.0) / ss;
-  T ak1 = A1 * ss1, ak2 = A2 * ss1, ak3 = A3 * ss1;
-  T vs = madd(A3, w1, madd(A2, v1, A1 * u1));
+__device__ inline void split_sw(const SplitK<T> &K, T c1, T d1, T u1, T v1, T w1, T *fp, T *fm) {
+  const T tmp1 = K.sw1, tmp3 = K.sw3, ss = K.ss, ak1 = K.ak1, ak2 = K.ak2, ak3 = K.ak3;
+  T vs = madd(K.A3, w1, madd(K.A2, v1, K.A1 * u1));
   T E1 = vs, E2 = madd(-c1, ss, vs), E3 = madd(c1, ss, vs);
   T E1P = (E1 + fabs(E1)) * T(0.5), E2P = (E2 + fabs(E2)) * T(0.5), E3P = (E3 + fabs(E3)) * T(0.5);
   T E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
-  T tmp0 = d1 / (T(2.0) * gamma);
+  T tmp0 = d1 / K.g2;
   T uc1 = madd(-c1, ak1, u1), uc2 = madd(c1, ak1, u1);
   T vc1 = madd(-c1, ak2, v1), vc2 = madd(c1, ak2, v1);
   T wc1 = madd(-c1, ak3, w1), wc2 = madd(c1, ak3, w1);
   T vvc1 = madd(wc1, wc1, madd(vc1, vc1, uc1 * uc1)) * T(0.5);
   T vvc2 = madd(wc2, wc2, madd(vc2, vc2, uc2 * uc2)) * T(0.5);
-  T vv = (gamma - T(1.0)) * madd(w1, w1, madd(v1, v1, u1 * u1));
+  T vv = K.gm1 * madd(w1, w1, madd(v1, v1, u1 * u1));
   T W2 = tmp3 * c1 * c1;
   fp[0] = tmp0 * (madd(tmp1, E1P, E2P) + E3P);
   fp[1] = tmp0 * madd(E3P, uc2, madd(E2P, uc1, tmp1 * E1P * u1));
@@ -520,10 +550,8 @@ __device__ inline void split_sw(T gamma, T c1, T d1, T u1, T v1, T w1, T A1,
 
 // split_Local_LaxFriedrichs (OCFD_flux_split.f90:73-104)
 template <typename T>
-__device__ inline void split_llf(T gamma, T c1, T d1, T u1, T v1, T w1,
-                                 T A1, T A2, T A3, T *fp, T *fm) {
-  const T tmp0 = T(1.0) / gamma, tmp1 = T(1.0) / (gamma - T(1.0));
-  T ss = sqrt(madd(A3, A3, madd(A2, A2, A1 * A1)));
+__device__ inline void split_llf(const SplitK<T> &K, T c1, T d1, T u1, T v1, T w1, T *fp, T *fm) {
+  const T tmp0 = K.llf0, tmp1 = K.llf1, ss = K.ss, A1 = K.A1, A2 = K.A2, A3 = K.A3;
   T un = madd(A3, w1, madd(A2, v1, A1 * u1));
   T lamda0 = madd(ss, c1, fabs(un));
   T p1 = tmp0 * d1 * c1 * c1;

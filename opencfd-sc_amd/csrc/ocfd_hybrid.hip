@@ -108,6 +108,7 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   const int nbatch = (count + FB - 1) / FB;
   const int *list = B.hlist[AXIS];
   double *out = B.wflux[AXIS];
+  const bool sw = S.flux_split == 1;
   for (int batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
     const int f0 = batch * FB;
     const int nf = count - f0 < FB ? count - f0 : FB;
@@ -119,15 +120,16 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
       const int64_t h = TI::hidx(g, s - 3 + p, w, c);
       const double dd = B.d[h], uu = B.u[h], vv = B.v[h], ww = B.w[h];
       const double cc = sqrt(B.T[h]) / rma;
-      double A1, A2, A3;
-      if (FAST || B.M.uniform) {
-        A1 = B.M.u[q1]; A2 = B.M.u[q1 + 1]; A3 = B.M.u[q1 + 2];
-      } else {
-        A1 = B.M.p[q1][h]; A2 = B.M.p[q1 + 1][h]; A3 = B.M.p[q1 + 2][h];
-      }
+      // the splitting's constants per point (held across the batch loop they
+      // would cost registers, i.e. occupancy)
+      SplitK<double> K;
+      split_k_gamma(gamma, K);
+      if (FAST || B.M.uniform) split_k_metric(B.M.u[q1], B.M.u[q1 + 1], B.M.u[q1 + 2], sw, K);
+      else split_k_metric(B.M.p[q1][h], B.M.p[q1 + 1][h], B.M.p[q1 + 2][h], sw, K);
+      const double A1 = K.A1, A2 = K.A2, A3 = K.A3;
       double fp[5], fm[5];
-      if (S.flux_split == 1) split_sw(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
-      else split_llf(gamma, cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+      if (sw) split_sw(K, cc, dd, uu, vv, ww, fp, fm);
+      else split_llf(K, cc, dd, uu, vv, ww, fp, fm);
 #pragma unroll
       for (int m = 0; m < 5; m++) {
         sF[fb][p * 10 + m] = fp[m];
@@ -234,19 +236,37 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   }
 }
 
+// The grid of the striding face kernel: as many workgroups as the device
+// holds at once (CUs x resident workgroups per CU), so that every workgroup
+// runs from the start and the batches spread evenly; a larger grid leaves a
+// tail of workgroups that start late and run nearly alone.  `cache` holds
+// the value per device for one kernel.
+template <typename K>
+dim3 resident_grid(K kernel, int (&cache)[64]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cache[dev] == 0) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, NT, 0) != hipSuccess || per < 1) per = 1;
+    cache[dev] = cus * per;
+  }
+  return dim3((unsigned)cache[dev]);
+}
+
+template <int AXIS, bool CHAR, bool FAST>
+hipError_t weno_launch(const Block &B, hipStream_t st) {
+  static int cache[64] = {};
+  hipLaunchKernelGGL((k_weno_faces<AXIS, CHAR, FAST>), resident_grid(k_weno_faces<AXIS, CHAR, FAST>, cache),
+                     dim3(NT), 0, st, B);
+  return hipGetLastError();
+}
+
 template <int AXIS>
 hipError_t weno_axis(const Block &B, hipStream_t st) {
-  // enough workgroups to fill the chip; each strides over the batches
-  const dim3 grid(1024);
   const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
-  if (B.S.is_char) {
-    if (fast) hipLaunchKernelGGL((k_weno_faces<AXIS, true, true>), grid, dim3(NT), 0, st, B);
-    else hipLaunchKernelGGL((k_weno_faces<AXIS, true, false>), grid, dim3(NT), 0, st, B);
-  } else {
-    if (fast) hipLaunchKernelGGL((k_weno_faces<AXIS, false, true>), grid, dim3(NT), 0, st, B);
-    else hipLaunchKernelGGL((k_weno_faces<AXIS, false, false>), grid, dim3(NT), 0, st, B);
-  }
-  return hipGetLastError();
+  if (B.S.is_char) return fast ? weno_launch<AXIS, true, true>(B, st) : weno_launch<AXIS, true, false>(B, st);
+  return fast ? weno_launch<AXIS, false, true>(B, st) : weno_launch<AXIS, false, false>(B, st);
 }
 
 }  // namespace

@@ -113,22 +113,26 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   R *sH = sQ + (PROJ ? nq * NP * TW : 0);          // face fluxes       [NH][NF][TW]
   int *sS = reinterpret_cast<int *>(sH + NH * NF * TW);  // hybrid: list slot of each face [NF][TW]
   // ---- phase 1: split fluxes at the tile's points ----
+  // the splitting's constants once per thread (and the metric part once per
+  // sweep on a uniform grid)
+  const bool sw = S.flux_split == 1;
+  SplitK<R> K;
+  split_k_gamma(R(gamma), K);
+  if (unif) split_k_metric(R(B.M.u[q1]), R(B.M.u[q1 + 1]), R(B.M.u[q1 + 2]), sw, K);
 #pragma unroll
   for (int r = 0; r < NI1; r++) {
     int p, wl;
     if (!p1(r, p, wl)) continue;
     const R dd = R(pq[r][0]), uu = R(pq[r][1]), vv = R(pq[r][2]), ww = R(pq[r][3]);
     const R cc = sqrt(R(pq[r][4])) / R(rma);  // OCFD_inviscous.f90:67
-    R A1, A2, A3;
-    if (unif) {
-      A1 = R(B.M.u[q1]); A2 = R(B.M.u[q1 + 1]); A3 = R(B.M.u[q1 + 2]);
-    } else {
+    if (!unif) {
       const int64_t h = TI::hidx(g, s0 - 4 + p, w0 + wl, c);
-      A1 = R(B.M.p[q1][h]); A2 = R(B.M.p[q1 + 1][h]); A3 = R(B.M.p[q1 + 2][h]);
+      split_k_metric(R(B.M.p[q1][h]), R(B.M.p[q1 + 1][h]), R(B.M.p[q1 + 2][h]), sw, K);
     }
+    const R A1 = K.A1, A2 = K.A2, A3 = K.A3;
     R fp[5], fm[5];
-    if (S.flux_split == 1) split_sw<R>(R(gamma), cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
-    else split_llf<R>(R(gamma), cc, dd, uu, vv, ww, A1, A2, A3, fp, fm);
+    if (sw) split_sw<R>(K, cc, dd, uu, vv, ww, fp, fm);
+    else split_llf<R>(K, cc, dd, uu, vv, ww, fp, fm);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       sF[lidx<AXIS, NP, TW>(m, p, wl)] = fp[m];
