@@ -432,7 +432,8 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   if (cfg->nfiltering > 0) rc |= alloc(&h->fbuf, fb);
   rc |= alloc(&B.trash, 16 * 256 * sizeof(double));
   for (int a = 0; a < 3; a++) {
-    B.hlist[a] = B.hmap[a] = nullptr;
+    B.hlist[a] = B.hgbase[a] = nullptr;
+    B.hbits[a] = nullptr;
     B.wflux[a] = nullptr;
   }
   B.hcount = nullptr;
@@ -442,7 +443,9 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
     const int64_t nfa[3] = {TileIndex<0>::nfaces(B.g), TileIndex<1>::nfaces(B.g), TileIndex<2>::nfaces(B.g)};
     for (int a = 0; a < 3; a++) {
       rc |= alloc((double **)&B.hlist[a], (size_t)nfa[a] * sizeof(int));
-      rc |= alloc((double **)&B.hmap[a], (size_t)nfa[a] * sizeof(int));
+      const int64_t ng = (nfa[a] + 63) / 64;
+      rc |= alloc((double **)&B.hbits[a], (size_t)ng * sizeof(unsigned long long));
+      rc |= alloc((double **)&B.hgbase[a], (size_t)ng * sizeof(int));
       rc |= alloc(&B.wflux[a], (size_t)nfa[a] * nh * sizeof(double));
     }
     rc |= alloc((double **)&B.hcount, 4 * sizeof(int));
@@ -476,8 +479,10 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   if (B.hcount) {
     // no face listed until the first comput_Rhybrid
     for (int a = 0; a < 3; a++) {
-      const int64_t nfa = a == 0 ? TileIndex<0>::nfaces(B.g) : (a == 1 ? TileIndex<1>::nfaces(B.g) : TileIndex<2>::nfaces(B.g));
-      hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nfa + 255) / 256)), dim3(256), 0, h->st, B.hmap[a], nfa, -1);
+      const int64_t ng = a == 0 ? TileIndex<0>::ngroups(B.g) : (a == 1 ? TileIndex<1>::ngroups(B.g) : TileIndex<2>::ngroups(B.g));
+      hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((2 * ng + 255) / 256)), dim3(256), 0, h->st, (int *)B.hbits[a],
+                         2 * ng, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, h->st, B.hgbase[a], ng, 0);
     }
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, B.hcount, 4);
   }
@@ -505,7 +510,8 @@ int ocfd_destroy(ocfd_handle *h) {
     if (p) (void)hipFree(p);
   for (int a = 0; a < 3; a++) {
     if (B.hlist[a]) (void)hipFree(B.hlist[a]);
-    if (B.hmap[a]) (void)hipFree(B.hmap[a]);
+    if (B.hbits[a]) (void)hipFree(B.hbits[a]);
+    if (B.hgbase[a]) (void)hipFree(B.hgbase[a]);
     if (B.wflux[a]) (void)hipFree(B.wflux[a]);
   }
   if (B.hcount) (void)hipFree(B.hcount);

@@ -83,7 +83,11 @@ __global__ __launch_bounds__(NT) void k_hyb_list(const Block B) {
     const bool weno = (wbits >> it) & 1u;
     const unsigned long long mask = __ballot(weno);
     const int slot = base + soff[it][wave] + __popcll(mask & below);
-    if (fid < nfa) B.hmap[AXIS][fid] = weno ? slot : -1;
+    // the wavefront's 64 faces are one aligned group of the face map
+    if (lane == 0 && fid < nfa) {
+      B.hbits[AXIS][fid >> 6] = mask;
+      B.hgbase[AXIS][fid >> 6] = base + soff[it][wave];
+    }
     if (weno) B.hlist[AXIS][slot] = (int)(fid * 2 + ((hbits >> it) & 1u));
   }
 }

@@ -44,9 +44,14 @@ struct Block {
   double *ev;                 // 12 interior planes: Ev1(1:4), Ev2(1:4), Ev3(1:4)
   double *trash;              // 16 x 256 doubles: stores of lanes outside the block
   // hybrid scheme, per sweep axis: the faces that use WENO7/WENO5 (face id
-  // * 2 + (WENO5)), their count, the face -> list slot map (-1: linear) and
-  // the fluxes of the listed faces (5 per face characteristic, 10 otherwise)
-  int *hlist[3], *hmap[3], *hcount;
+  // * 2 + (WENO5)), their count, the face -> list slot map and the fluxes of
+  // the listed faces (5 per face characteristic, 10 otherwise).  The map is
+  // kept per group of 64 consecutive face ids: a bit mask of the listed
+  // faces and the slot of the group's first listed face (slots follow the
+  // face order), i.e. 12 bytes per 64 faces.
+  int *hlist[3], *hcount;
+  unsigned long long *hbits[3];
+  int *hgbase[3];
   double *wflux[3];
   int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
   int kseg;  // boxes of a batch: planes per workgroup of the k-marching kernels
@@ -116,6 +121,7 @@ struct TileIndex {
   __host__ __device__ static inline int64_t nfaces(const Geom &g) {
     return (int64_t)(g.n(AXIS) + 1) * nwidth(g) * nthird(g);
   }
+  __host__ __device__ static inline int64_t ngroups(const Geom &g) { return (nfaces(g) + 63) / 64; }
   __device__ static inline int64_t face_id(const Geom &g, int s, int w, int c) {
     if (AXIS == 0) return ((int64_t)(c - 1) * g.ny + (w - 1)) * (g.nx + 1) + s;
     return ((int64_t)(c - 1) * (g.n(AXIS) + 1) + s) * g.nx + (w - 1);

@@ -157,7 +157,12 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
       if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
       const int s = s0 - 1 + f, w = w0 + wl;
       int slot = -1;
-      if (s <= n && w <= nw) slot = B.hmap[AXIS][TI::face_id(g, s, w, c)];
+      if (s <= n && w <= nw) {
+        const int64_t fid = TI::face_id(g, s, w, c);
+        const unsigned long long bits = B.hbits[AXIS][fid >> 6];
+        const int bit = (int)(fid & 63);
+        if ((bits >> bit) & 1ull) slot = B.hgbase[AXIS][fid >> 6] + __popcll(bits & ((1ull << bit) - 1ull));
+      }
       sS[lidx<AXIS, NF, TW>(0, f, wl)] = slot;
       if (slot >= 0) {
         const double *wf = B.wflux[AXIS] + (int64_t)slot * NH;
