@@ -220,7 +220,7 @@ def profile_traffic(workload: str, precision: str, name: str):
         return None
     best = None
     for line in open(path):
-        if line.startswith("Occupancy"):
+        if line.startswith("Occupancy and issue"):
             break  # the per-kernel HBM table ends where the occupancy table starts
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) < 7 or not cols[0].startswith("`" + sym):
