@@ -109,9 +109,16 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   // the characteristic inputs are needed only where the sweep itself
   // projects, i.e. not for the hybrid scheme (its WENO faces come precomputed)
   constexpr bool PROJ = CHAR && !HYB;
+  // FP64 tiles are held to 4 workgroups per CU by their LDS: there the face
+  // fluxes take the place of the split fluxes once phase 2 has read them
+  // (phase 2 keeps its results in registers across one more barrier).  The
+  // FP32 tiles, bound by instruction issue, keep separate arrays and the
+  // plain item loops (measured faster for them).
+  constexpr bool ALIAS = sizeof(R) == 8;
   R *sQ = sF + 10 * NP * TW;                       // char inputs       [nq][NP][TW]
-  R *sH = sQ + (PROJ ? nq * NP * TW : 0);          // face fluxes       [NH][NF][TW]
-  int *sS = reinterpret_cast<int *>(sH + NH * NF * TW);  // hybrid: list slot of each face [NF][TW]
+  R *const sQend = sQ + (PROJ ? nq * NP * TW : 0);
+  R *sH = ALIAS ? sF : sQend;                      // face fluxes       [NH][NF][TW]
+  int *sS = reinterpret_cast<int *>(ALIAS ? sQend : sH + NH * NF * TW);  // hybrid: list slot of each face [NF][TW]
   // ---- phase 1: split fluxes at the tile's points ----
   // the splitting's constants once per thread (and the metric part once per
   // sweep on a uniform grid)
@@ -151,7 +158,8 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     }
   }
   if (HYB) {
-    // WENO faces of the hybrid scheme come precomputed (k_weno_faces)
+    // WENO faces of the hybrid scheme come precomputed (k_weno_faces); with
+    // ALIAS phase 2 reads them, as the face array is not free yet
     for (int it = tid; it < NF * TW; it += NTHREADS) {
       int f, wl;
       if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
@@ -164,7 +172,7 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
         if ((bits >> bit) & 1ull) slot = B.hgbase[AXIS][fid >> 6] + __popcll(bits & ((1ull << bit) - 1ull));
       }
       sS[lidx<AXIS, NF, TW>(0, f, wl)] = slot;
-      if (slot >= 0) {
+      if (!ALIAS && slot >= 0) {
         const double *wf = B.wflux[AXIS] + (int64_t)slot * NH;
 #pragma unroll
         for (int m = 0; m < NH; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = R(wf[m]);
@@ -184,15 +192,15 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
 
   // ---- phase 2: face fluxes ----
   // flux+/- of component m at face f (non-characteristic)
-  auto face_nc = [&](int m, int f, int wl, int s, int scm) {
+  auto face_nc = [&](int m, int f, int wl, int s, int scm, R &hp, R &hm) {
     R xp[8], xm[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       xp[k] = sF[lidx<AXIS, NP, TW>(m, f + k, wl)];
       xm[k] = sF[lidx<AXIS, NP, TW>(5 + m, f + k, wl)];
     }
-    sH[lidx<AXIS, NF, TW>(m, f, wl)] = FAST ? inner_p<SCH>(S, xp, scm) : face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
-    sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] = FAST ? inner_m<SCH>(S, xm, scm) : face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
+    hp = FAST ? inner_p<SCH>(S, xp, scm) : face_p_nonchar<SCH>(S, xp, s, n, AXIS, scm);
+    hm = FAST ? inner_m<SCH>(S, xm, scm) : face_m_nonchar<SCH>(S, xm, s, n, AXIS, scm);
   };
 #define FPV(m, k) sF[lidx<AXIS, NP, TW>((m), f + (k), wl)]
 #define FMV(m, k) sF[lidx<AXIS, NP, TW>(5 + (m), f + (k), wl)]
@@ -276,14 +284,12 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   };
   // all five components of face f, projected on the characteristic basis of
   // the face (flux_charteric, OCFD_inviscous_character.f90:190-290)
-  auto face_ch = [&](int f, int wl, int s, int scm) {
+  auto face_ch = [&](int f, int wl, int s, int scm, R *hh) {
     CharBasisT<R> Bs;
     basis_of(f, wl, Bs);
-    R h0[5], hh[5];
+    R h0[5];
     for (int r = 0; r < 5; r++) h0[r] = char_row(Bs, f, wl, s, scm, r);
     char_back(Bs, h0, hh);
-#pragma unroll
-    for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = hh[m];
   };
   auto listed = [&](int f, int wl) { return sS[lidx<AXIS, NF, TW>(0, f, wl)] >= 0; };
   // face work item -> (component, face, line)
@@ -295,39 +301,115 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     if (AXIS == 0) { f = it % NF; wl = it / NF; } else { wl = it % TW; f = it / TW; }
   };
   const bool closure_lo = !FAST && lo == 1 && sl == 0, closure_hi = !FAST && hi == 1 && sr == 0;
-  if (!CHAR) {
-    for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
-      int m, f, wl;
-      item_nc(it, m, f, wl);
-      const int s = s0 - 1 + f;
-      if (s > n || w0 + wl > nw) continue;
-      if (HYB && listed(f, wl)) continue;
-      face_nc(m, f, wl, s, HYB ? 1 : 0);
+  auto slot_of = [&](int f, int wl) { return sS[lidx<AXIS, NF, TW>(0, f, wl)]; };
+  // work items of this thread: (component, face, line), or (face, line)
+  // on the characteristic path of the non-hybrid schemes
+  constexpr int NIT = PROJ ? NF * TW : 5 * NF * TW;
+  constexpr int NI2 = (NIT + NTHREADS - 1) / NTHREADS;
+  constexpr int NV = CHAR ? (HYB ? 1 : 5) : 2;  // face-array values per item
+  R hv[ALIAS ? NI2 : 1][NV];
+  // item r -> its face-array entries; false for items outside the tile
+  auto decode = [&](int r, int &m, int &f, int &wl) {
+    const int it = tid + r * NTHREADS;
+    if (it >= NIT) return false;
+    if (PROJ) { m = 0; item_ch(it, f, wl); } else item_nc(it, m, f, wl);
+    return s0 - 1 + f <= n && w0 + wl <= nw;
+  };
+  auto put = [&](int m, int f, int wl, const R *v) {
+    if constexpr (NV == 2) {
+      sH[lidx<AXIS, NF, TW>(m, f, wl)] = v[0];
+      sH[lidx<AXIS, NF, TW>(5 + m, f, wl)] = v[1];
+    } else if constexpr (NV == 5) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) sH[lidx<AXIS, NF, TW>(q, f, wl)] = v[q];
+    } else {
+      sH[lidx<AXIS, NF, TW>(m, f, wl)] = v[0];
     }
-  } else if (!HYB) {
-    for (int it = tid; it < NF * TW; it += NTHREADS) {
-      int f, wl;
-      item_ch(it, f, wl);
+  };
+  if constexpr (ALIAS) {
+    // results held in registers until every stencil has been read
+#pragma unroll
+    for (int r = 0; r < NI2; r++) {
+      int m, f, wl;
+      if (!decode(r, m, f, wl)) continue;
       const int s = s0 - 1 + f;
-      if (s > n || w0 + wl > nw) continue;
-      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
-        for (int m = 0; m < 5; m++) sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
-        continue;
+      R v[NV];
+      if constexpr (!CHAR) {
+        if (HYB && slot_of(f, wl) >= 0) {
+          const double *wf = B.wflux[AXIS] + (int64_t)slot_of(f, wl) * NH;
+          v[0] = R(wf[m]);
+          v[1] = R(wf[5 + m]);
+        } else {
+          face_nc(m, f, wl, s, HYB ? 1 : 0, v[0], v[1]);
+        }
+      } else if constexpr (!HYB) {
+        if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+#pragma unroll
+          for (int q = 0; q < NV; q++) v[q] = closure_comp(f, wl, s, q);
+        } else {
+          face_ch(f, wl, s, 0, v);
+        }
+      } else {
+        // hybrid, characteristic: the linear and closure faces, per component
+        if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+          v[0] = closure_comp(f, wl, s, m);
+        } else if (slot_of(f, wl) >= 0) {
+          v[0] = R(B.wflux[AXIS][(int64_t)slot_of(f, wl) * NH + m]);
+        } else {
+          v[0] = linear_comp(f, wl, s, m);
+        }
       }
-      face_ch(f, wl, s, 0);
+#pragma unroll
+      for (int q = 0; q < NV; q++) hv[r][q] = v[q];
+    }
+    // every stencil has been read: the face fluxes go over the split fluxes
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NI2; r++) {
+      int m, f, wl;
+      if (!decode(r, m, f, wl)) continue;
+      put(m, f, wl, hv[r]);
     }
   } else {
-    // hybrid, characteristic: the linear and closure faces, per component
-    for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
-      int m, f, wl;
-      item_nc(it, m, f, wl);
-      const int s = s0 - 1 + f;
-      if (s > n || w0 + wl > nw) continue;
-      if ((closure_lo && s == 0) || (closure_hi && s == n)) {
-        sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
-        continue;
+    // separate face array: each item's fluxes are stored as computed
+    if constexpr (!CHAR) {
+      for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
+        int m, f, wl;
+        item_nc(it, m, f, wl);
+        const int s = s0 - 1 + f;
+        if (s > n || w0 + wl > nw) continue;
+        if (HYB && slot_of(f, wl) >= 0) continue;  // written in phase 1
+        R v[2];
+        face_nc(m, f, wl, s, HYB ? 1 : 0, v[0], v[1]);
+        put(m, f, wl, v);
       }
-      if (!listed(f, wl)) sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+    } else if constexpr (!HYB) {
+      for (int it = tid; it < NF * TW; it += NTHREADS) {
+        int f, wl;
+        item_ch(it, f, wl);
+        const int s = s0 - 1 + f;
+        if (s > n || w0 + wl > nw) continue;
+        R v[5];
+        if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+          for (int m = 0; m < 5; m++) v[m] = closure_comp(f, wl, s, m);
+        } else {
+          face_ch(f, wl, s, 0, v);
+        }
+        put(0, f, wl, v);
+      }
+    } else {
+      // hybrid, characteristic: the linear and closure faces, per component
+      for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
+        int m, f, wl;
+        item_nc(it, m, f, wl);
+        const int s = s0 - 1 + f;
+        if (s > n || w0 + wl > nw) continue;
+        if ((closure_lo && s == 0) || (closure_hi && s == n)) {
+          sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
+          continue;
+        }
+        if (slot_of(f, wl) < 0) sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+      }
     }
   }
 #undef FPV
@@ -385,9 +467,12 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid_batch(const BoxSet S, con
   sweep_tile<AXIS, SCH, CHAR, TL, TW, true, R>(S.blk[S.ids[j]], t - S.off[j], mode);
 }
 
+// LDS bytes of a sweep tile: split fluxes, characteristic inputs, face fluxes
+// (in the split fluxes' place for FP64 tiles), hybrid slots
 template <int TL, int TW, bool CHAR, bool HYB, typename R>
 constexpr size_t sweep_lds(int nq) {
-  return sizeof(R) * (10 * (TL + 8) * TW + (CHAR && !HYB ? nq * (TL + 8) * TW : 0) + (CHAR ? 5 : 10) * (TL + 1) * TW) +
+  return sizeof(R) * (10 * (TL + 8) * TW + (CHAR && !HYB ? nq * (TL + 8) * TW : 0) +
+                      (sizeof(R) == 8 ? 0 : (CHAR ? 5 : 10) * (TL + 1) * TW)) +
          (HYB ? sizeof(int) * (TL + 1) * TW : 0);
 }
 

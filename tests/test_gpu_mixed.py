@@ -107,3 +107,40 @@ def test_mixed_against_oracle():
     errs = [rel_l2(interior(pg[q]), interior(po[q])) for q in range(5)]
     print("relative L2 vs oracle after 3 steps:", " ".join("%.2e" % e for e in errs))
     assert max(errs) <= 1e-5, errs
+
+
+def test_mixed_primitive_storage_and_transfers(tmp_path):
+    """The mixed form stores the primitives in FP32.  Downloads return their
+    FP32 rounding, halos included; f formed at the upload comes from the FP64
+    values given, so it equals the FP64 path's f bit for bit; a snapshot
+    written from the mixed block holds the stored values, and reading it
+    back into a mixed block restores them and forms f from them as the FP64
+    path does from the same file."""
+    from ocfd import fortran_io
+    case = cases.random_case(shape=(20, 28, 40), seed=78, scheme="WENO5", split=1, char=0)
+    prims = case.halo_prims()
+
+    def bare(precision):
+        b = native.Block(case.cfg.replace(precision=precision))
+        if case.cfg.metric_mode == 1:
+            b.set_uniform_metrics(case.uniform_metrics)
+        else:
+            b.set_grid(case.x, case.y, case.z, case.spans)
+        b.upload_primitives(prims, 0, 0.0)
+        return b
+
+    with bare(0) as b64, bare(PREC_MIXED) as bmx:
+        pmx = bmx.download_primitives()
+        for q in range(5):
+            assert np.array_equal(pmx[q], prims[q].astype(np.float32).astype(np.float64)), "primitive %d" % q
+        assert np.array_equal(bmx.download_f(), b64.download_f()), "f formed from the FP64 upload"
+        path = str(tmp_path / "mixed.dat")
+        bmx.save_flow_data(path)
+        _, _, filed = fortran_io.read_flow_data(path, case.cfg.shape)
+        for q in range(5):
+            assert np.array_equal(filed[q], interior(pmx[q])), "snapshot field %d" % q
+        assert bmx.read_flow_data(path) is False and b64.read_flow_data(path) is False
+        back = bmx.download_primitives()
+        for q in range(5):
+            assert np.array_equal(back[q], pmx[q]), "read back %d" % q
+        assert np.array_equal(bmx.download_f(), b64.download_f()), "f formed at the restart"
