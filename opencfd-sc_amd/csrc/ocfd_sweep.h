@@ -302,17 +302,18 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   };
   const bool closure_lo = !FAST && lo == 1 && sl == 0, closure_hi = !FAST && hi == 1 && sr == 0;
   auto slot_of = [&](int f, int wl) { return sS[lidx<AXIS, NF, TW>(0, f, wl)]; };
-  // work items of this thread: (component, face, line), or (face, line)
-  // on the characteristic path of the non-hybrid schemes
-  constexpr int NIT = PROJ ? NF * TW : 5 * NF * TW;
+  // work items of this thread: (component, face, line), or on the
+  // characteristic path (face, line) with all five components, so that the
+  // item's index arithmetic, slot and closure tests are done once per face
+  constexpr int NIT = CHAR ? NF * TW : 5 * NF * TW;
   constexpr int NI2 = (NIT + NTHREADS - 1) / NTHREADS;
-  constexpr int NV = CHAR ? (HYB ? 1 : 5) : 2;  // face-array values per item
+  constexpr int NV = CHAR ? 5 : 2;  // face-array values per item
   R hv[ALIAS ? NI2 : 1][NV];
   // item r -> its face-array entries; false for items outside the tile
   auto decode = [&](int r, int &m, int &f, int &wl) {
     const int it = tid + r * NTHREADS;
     if (it >= NIT) return false;
-    if (PROJ) { m = 0; item_ch(it, f, wl); } else item_nc(it, m, f, wl);
+    if (CHAR) { m = 0; item_ch(it, f, wl); } else item_nc(it, m, f, wl);
     return s0 - 1 + f <= n && w0 + wl <= nw;
   };
   auto put = [&](int m, int f, int wl, const R *v) {
@@ -350,13 +351,17 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
           face_ch(f, wl, s, 0, v);
         }
       } else {
-        // hybrid, characteristic: the linear and closure faces, per component
+        // hybrid, characteristic: the closure, listed and linear faces
         if ((closure_lo && s == 0) || (closure_hi && s == n)) {
-          v[0] = closure_comp(f, wl, s, m);
+#pragma unroll
+          for (int q = 0; q < 5; q++) v[q] = closure_comp(f, wl, s, q);
         } else if (slot_of(f, wl) >= 0) {
-          v[0] = R(B.wflux[AXIS][(int64_t)slot_of(f, wl) * NH + m]);
+          const double *wf = B.wflux[AXIS] + (int64_t)slot_of(f, wl) * NH;
+#pragma unroll
+          for (int q = 0; q < 5; q++) v[q] = R(wf[q]);
         } else {
-          v[0] = linear_comp(f, wl, s, m);
+#pragma unroll
+          for (int q = 0; q < 5; q++) v[q] = linear_comp(f, wl, s, q);
         }
       }
 #pragma unroll
@@ -398,17 +403,24 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
         put(0, f, wl, v);
       }
     } else {
-      // hybrid, characteristic: the linear and closure faces, per component
-      for (int it = tid; it < 5 * NF * TW; it += NTHREADS) {
-        int m, f, wl;
-        item_nc(it, m, f, wl);
+      // hybrid, characteristic: the linear and closure faces, all five
+      // components per item
+      for (int it = tid; it < NF * TW; it += NTHREADS) {
+        int f, wl;
+        item_ch(it, f, wl);
         const int s = s0 - 1 + f;
         if (s > n || w0 + wl > nw) continue;
+        R v[5];
         if ((closure_lo && s == 0) || (closure_hi && s == n)) {
-          sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
-          continue;
+#pragma unroll
+          for (int q = 0; q < 5; q++) v[q] = closure_comp(f, wl, s, q);
+        } else if (slot_of(f, wl) < 0) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) v[q] = linear_comp(f, wl, s, q);
+        } else {
+          continue;  // written in phase 1
         }
-        if (slot_of(f, wl) < 0) sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+        put(0, f, wl, v);
       }
     }
   }
