@@ -172,6 +172,38 @@ class BlockBase:
     def bc(self):
         return self._call("bc")
 
+    def set_bc_boundarylayer(self, aoa=0.0, inlet=None, upper=None, nonref=None, pertb=None, wall=None):
+        """ocfd_set_bc_boundarylayer: the data of the boundary-layer /
+        swept-corner conditions -- inlet (5, nz, ny) and upper (5, nz, nx)
+        primitives, nonref and wall (nz, nx) flags, pertb (nz, nx); None
+        leaves an item out (free stream, all Dirichlet, no blowing, all
+        wall)."""
+        cfg = self.cfg
+
+        def dbl(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != shape:
+                raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+            return _ptr(a), a
+
+        def i32(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            if a.shape != shape:
+                raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+            return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), a
+
+        items = [dbl(inlet, (5, cfg.nz, cfg.ny)), dbl(upper, (5, cfg.nz, cfg.nx)), i32(nonref, (cfg.nz, cfg.nx)),
+                 dbl(pertb, (cfg.nz, cfg.nx)), i32(wall, (cfg.nz, cfg.nx))]
+        ptrs = [it[0] if it is not None else None for it in items]
+        return self._call("set_bc_boundarylayer", ctypes.c_double(aoa), *ptrs)
+
+    def set_bc_time_factor(self, ht):
+        return self._call("set_bc_time_factor", ctypes.c_double(ht))
+
     def filtering(self):
         return self._call("filtering")
 
