@@ -370,8 +370,9 @@ def main():
                                              ((32, 16, 64)[s], 256 // (32, 16, 64)[s])
                                              for a, s in enumerate(b.tiles)]]
     res["config"]["precision"] = ("FP64" if prec == PREC_FP64 else
-                                  "mixed: FP32 splitting, reconstruction, viscous derivatives and du; FP64 state, "
-                                  "RK3, metrics and the sensor-flagged WENO faces")
+                                  "mixed: FP32 splitting, reconstruction, viscous derivatives, du and stored "
+                                  "primitives; FP64 conserved variables, RK3, metrics and the characteristic "
+                                  "basis and projections of the sensor-flagged faces")
     res["kernels_ms_per_step"] = {k: round(v[0] / args.prof_steps, 4) for k, v in prof.items()}
     if args.calibrate_copy:
         # k_copy moves exactly 5*cells*8 bytes in and 5*cells*8 bytes out per launch

@@ -47,8 +47,9 @@ enum { OCFD_BC_NONE = 0, OCFD_BC_BLUNT2D = 1, OCFD_BC_BOUNDARYLAYER = 2, OCFD_BC
    stresses and fluxes and accumulates du in FP32, and stores the primitives
    d, u, v, w, T (derived values, recomputed from f every stage in FP64) as
    FP32; the conserved variables f, fn, the RK3 stage combination, metrics and
-   Jacobian stay FP64, and so do the characteristic projection and WENO
-   reconstruction of the faces the hybrid sensor flags.  The transfers of du
+   Jacobian stay FP64, and so do the characteristic basis and projections of
+   the faces the hybrid sensor flags (their splitting and WENO reconstruction
+   are FP32).  The transfers of du
    and of the primitives (upload, download, flow-data files) convert on the
    host; an upload of the primitives forms f from the FP64 values given. */
 enum { OCFD_PREC_FP64 = 0, OCFD_PREC_MIXED = 1 };

@@ -93,12 +93,15 @@ __global__ __launch_bounds__(NT) void k_hyb_list(const Block B) {
 }
 
 // Fluxes of the listed faces of one axis.  A fixed grid strides over batches
-// of FB faces; the list length is read from the device counter.
-template <int AXIS, bool CHAR, bool FAST>
+// of FB faces; the list length is read from the device counter.  RR is the
+// real type of the splitting and the WENO reconstruction: double, or float
+// in the mixed-precision form, where the characteristic basis and the
+// projections onto and back from it stay FP64.
+template <int AXIS, bool CHAR, bool FAST, typename RR>
 __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   typedef TileIndex<AXIS> TI;
   constexpr int NH = CHAR ? 5 : 10;
-  __shared__ double sF[FB][SFS];    // fp(1:5), fm(1:5) at stencil points s-3 .. s+4: [p * 10 + m]
+  __shared__ RR sF[FB][SFS];        // fp(1:5), fm(1:5) at stencil points s-3 .. s+4: [p * 10 + m]
   __shared__ double sQ[FB][2][7];   // u, v, w, c, A1..A3 at points s, s+1
   __shared__ double sB[FB][21];     // characteristic basis of the face
   __shared__ double sG[FB][5];      // characteristic fluxes h0(1:5)
@@ -122,18 +125,20 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
       int s, w, c;
       TI::face_of(g, list[f0 + fb] >> 1, s, w, c);
       const int64_t h = TI::hidx(g, s - 3 + p, w, c);
-      const double dd = B.d[h], uu = B.u[h], vv = B.v[h], ww = B.w[h];
-      const double cc = sqrt(B.T[h]) / rma;
+      const double dd = B.d[h], uu = B.u[h], vv = B.v[h], ww = B.w[h], Tt = B.T[h];
+      const double cc = sqrt(Tt) / rma;
+      const double A1 = (FAST || B.M.uniform) ? B.M.u[q1] : B.M.p[q1][h];
+      const double A2 = (FAST || B.M.uniform) ? B.M.u[q1 + 1] : B.M.p[q1 + 1][h];
+      const double A3 = (FAST || B.M.uniform) ? B.M.u[q1 + 2] : B.M.p[q1 + 2][h];
       // the splitting's constants per point (held across the batch loop they
       // would cost registers, i.e. occupancy)
-      SplitK<double> K;
-      split_k_gamma(gamma, K);
-      if (FAST || B.M.uniform) split_k_metric(B.M.u[q1], B.M.u[q1 + 1], B.M.u[q1 + 2], sw, K);
-      else split_k_metric(B.M.p[q1][h], B.M.p[q1 + 1][h], B.M.p[q1 + 2][h], sw, K);
-      const double A1 = K.A1, A2 = K.A2, A3 = K.A3;
-      double fp[5], fm[5];
-      if (sw) split_sw(K, cc, dd, uu, vv, ww, fp, fm);
-      else split_llf(K, cc, dd, uu, vv, ww, fp, fm);
+      SplitK<RR> K;
+      split_k_gamma(RR(gamma), K);
+      split_k_metric(RR(A1), RR(A2), RR(A3), sw, K);
+      const RR ccr = sqrt(RR(Tt)) / RR(rma);  // OCFD_inviscous.f90:67, in the splitting's type
+      RR fp[5], fm[5];
+      if (sw) split_sw<RR>(K, ccr, RR(dd), RR(uu), RR(vv), RR(ww), fp, fm);
+      else split_llf<RR>(K, ccr, RR(dd), RR(uu), RR(vv), RR(ww), fp, fm);
 #pragma unroll
       for (int m = 0; m < 5; m++) {
         sF[fb][p * 10 + m] = fp[m];
@@ -184,18 +189,19 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         // stencils of the hybrid's WENO7 / WENO5 faces (Ka1_H2.. / Ka1_H3..)
         const int a1h = scm == 2 ? -3 : -2, b1h = scm == 2 ? 3 : 2;
         const int a2h = scm == 2 ? -2 : -1, b2h = scm == 2 ? 4 : 3;
+        // projection in FP64, reconstruction in RR
         double xp[8], xm[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          xp[k] = sr5[0] * sF[fb][k * 10];
-          xm[k] = sr5[0] * sF[fb][k * 10 + 5];
+          xp[k] = sr5[0] * double(sF[fb][k * 10]);
+          xm[k] = sr5[0] * double(sF[fb][k * 10 + 5]);
         }
 #pragma unroll
         for (int cmp = 1; cmp < 5; cmp++) {
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            xp[k] = madd(sr5[cmp], sF[fb][k * 10 + cmp], xp[k]);
-            xm[k] = madd(sr5[cmp], sF[fb][k * 10 + 5 + cmp], xm[k]);
+            xp[k] = madd(sr5[cmp], double(sF[fb][k * 10 + cmp]), xp[k]);
+            xm[k] = madd(sr5[cmp], double(sF[fb][k * 10 + 5 + cmp]), xm[k]);
           }
         }
 #pragma unroll
@@ -203,9 +209,22 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
           if (k < a1h + 3 || k > b1h + 3) xp[k] = 0.0;
           if (k < a2h + 3 || k > b2h + 3) xm[k] = 0.0;
         }
-        const double h1 = FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_char<SCH_HYBRID>(S, xp, s, n, AXIS, scm);
-        const double h2 = FAST ? inner_m<SCH_HYBRID>(S, xm, scm) : face_m_char<SCH_HYBRID>(S, xm, s, n, AXIS, scm);
-        sG[fb][r] = h1 + h2;
+        auto recon = [&](const auto *yp, const auto *ym) {
+          const auto h1 = FAST ? inner_p<SCH_HYBRID>(S, yp, scm) : face_p_char<SCH_HYBRID>(S, yp, s, n, AXIS, scm);
+          const auto h2 = FAST ? inner_m<SCH_HYBRID>(S, ym, scm) : face_m_char<SCH_HYBRID>(S, ym, s, n, AXIS, scm);
+          return double(h1 + h2);
+        };
+        if constexpr (sizeof(RR) == 8) {
+          sG[fb][r] = recon(xp, xm);
+        } else {
+          RR yp[8], ym[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            yp[k] = RR(xp[k]);
+            ym[k] = RR(xm[k]);
+          }
+          sG[fb][r] = recon(yp, ym);
+        }
       }
       __syncthreads();
       // back to conservative components (S1 * h0): one (face, component)
@@ -225,15 +244,16 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
         const int scm = (code & 1) ? 3 : 2;
         int s, w, c;
         TI::face_of(g, code >> 1, s, w, c);
-        double xp[8], xm[8];
+        RR xp[8], xm[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
           xp[k] = sF[fb][k * 10 + m];
           xm[k] = sF[fb][k * 10 + 5 + m];
         }
         double *o = out + (int64_t)(f0 + fb) * NH;
-        o[m] = FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_nonchar<SCH_HYBRID>(S, xp, s, n, AXIS, scm);
-        o[5 + m] = FAST ? inner_m<SCH_HYBRID>(S, xm, scm) : face_m_nonchar<SCH_HYBRID>(S, xm, s, n, AXIS, scm);
+        o[m] = double(FAST ? inner_p<SCH_HYBRID>(S, xp, scm) : face_p_nonchar<SCH_HYBRID>(S, xp, s, n, AXIS, scm));
+        o[5 + m] = double(FAST ? inner_m<SCH_HYBRID>(S, xm, scm)
+                               : face_m_nonchar<SCH_HYBRID>(S, xm, s, n, AXIS, scm));
       }
     }
     __syncthreads();  // the next batch overwrites the LDS arrays
@@ -258,19 +278,27 @@ dim3 resident_grid(K kernel, int (&cache)[64]) {
   return dim3((unsigned)cache[dev]);
 }
 
-template <int AXIS, bool CHAR, bool FAST>
+template <int AXIS, bool CHAR, bool FAST, typename RR>
 hipError_t weno_launch(const Block &B, hipStream_t st) {
   static int cache[64] = {};
-  hipLaunchKernelGGL((k_weno_faces<AXIS, CHAR, FAST>), resident_grid(k_weno_faces<AXIS, CHAR, FAST>, cache),
+  hipLaunchKernelGGL((k_weno_faces<AXIS, CHAR, FAST, RR>), resident_grid(k_weno_faces<AXIS, CHAR, FAST, RR>, cache),
                      dim3(NT), 0, st, B);
   return hipGetLastError();
 }
 
+template <int AXIS, typename RR>
+hipError_t weno_axis_as(const Block &B, hipStream_t st) {
+  const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
+  if (B.S.is_char)
+    return fast ? weno_launch<AXIS, true, true, RR>(B, st) : weno_launch<AXIS, true, false, RR>(B, st);
+  return fast ? weno_launch<AXIS, false, true, RR>(B, st) : weno_launch<AXIS, false, false, RR>(B, st);
+}
+
+// FP32 splitting and reconstruction in the mixed-precision form
 template <int AXIS>
 hipError_t weno_axis(const Block &B, hipStream_t st) {
-  const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
-  if (B.S.is_char) return fast ? weno_launch<AXIS, true, true>(B, st) : weno_launch<AXIS, true, false>(B, st);
-  return fast ? weno_launch<AXIS, false, true>(B, st) : weno_launch<AXIS, false, false>(B, st);
+  if (B.prec == OCFD_PREC_MIXED_CODE) return weno_axis_as<AXIS, float>(B, st);
+  return weno_axis_as<AXIS, double>(B, st);
 }
 
 }  // namespace
