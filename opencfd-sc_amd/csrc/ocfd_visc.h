@@ -12,8 +12,8 @@
 //               T <= 0 trap (whole-step path; du is not written back)
 //
 // k_visc_flux and k_visc_div_update march along k over a 64x4 tile of (i,j)
-// columns and a segment of up to 32 planes (fewer on small blocks so the
-// launch still fills the chip).  The x and y neighbours of the current plane
+// columns and a segment of up to 64 planes in FP64, 32 in FP32 (fewer on
+// small blocks so the launch still fills the chip).  The x and y neighbours of the current plane
 // come from a double-buffered LDS tile: while one plane is computed, every
 // lane already holds its share of the next plane's tile in registers, so
 // there is one barrier per plane and the loads overlap the arithmetic.  The
@@ -61,7 +61,7 @@ __device__ inline R di(const Block &B, const R *a, int axis, int i, int j, int k
   return dxyz0_point(at, s, n, B.P.dc[axis], B.S.scheme_vis == SCH_CD8, B.S.iperiodic[axis] == 1);
 }
 
-constexpr int TXT = 64, TYT = 4, KSEG_MAX = 32;
+constexpr int TXT = 64, TYT = 4, KSEG_MAX = 64;
 
 // Plane tile geometry: stencil radius R of the central difference, tile of
 // LX x LY points around the TXT x TYT columns.
@@ -425,7 +425,7 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
     return c;
   };
   CellVals cv = fetch_cell(tc.kb);
-  unsigned negmask = 0;
+  unsigned long long negmask = 0;  // bit k - kb: T <= 0 at plane k
 #pragma unroll 1
   for (int k = tc.kb; k <= tc.ke; k++) {
     const int64_t t = col + (int64_t)(k - 1) * skz;
@@ -488,7 +488,7 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
       __builtin_nontemporal_store(RT(w), tc.valid ? B.w.as<RT>() + ph : pt + 3 * NT);
       __builtin_nontemporal_store(RT(T), tc.valid ? B.T.as<RT>() + ph : pt + 4 * NT);
     }
-    negmask |= (tc.valid && T <= 0) ? (1u << (k - tc.kb)) : 0u;
+    negmask |= (tc.valid && T <= 0) ? (1ull << (k - tc.kb)) : 0ull;
     if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz))
       refresh_of(B.P.cv, d, u, v, w, T, fq);
 #pragma unroll
@@ -507,7 +507,7 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
   }
   // T <= 0 cells of this column, in k order
   while (negmask) {
-    const int kk = __builtin_ctz(negmask);
+    const int kk = __builtin_ctzll(negmask);
     negmask &= negmask - 1;
     const int slot = atomicAdd(neg, 1);
     if (slot < 64) neg[1 + slot] = (int)(col + (int64_t)(tc.kb + kk - 1) * skz);
@@ -551,12 +551,16 @@ __global__ __launch_bounds__(NT) void k_visc_div_update_batch(const BoxSet S, co
 // ---------------------------------------------------------------------------
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
 
-// planes per workgroup: the longest segment (<= 32) that still gives about
-// 2048 workgroups
+// planes per workgroup: the longest segment that still gives about 2048
+// workgroups, at most 32 planes, for the FP32 kernels; about 1024 workgroups
+// of at most 64 planes for the FP64 ones, which run fewer waves per SIMD and
+// gain from amortising the prologue of their plane queue over a longer march
+template <typename RT>
 inline int march_kseg(const Geom &g) {
   const int64_t cols = (int64_t)((g.nx + TXT - 1) / TXT) * ((g.ny + TYT - 1) / TYT);
-  int64_t k = cols * g.nz / 2048;
-  return (int)(k < 1 ? 1 : (k > KSEG_MAX ? KSEG_MAX : k));
+  const int64_t k = cols * g.nz / (sizeof(RT) == 8 ? 1024 : 2048);
+  const int kmax = sizeof(RT) == 8 ? KSEG_MAX : 32;
+  return (int)(k < 1 ? 1 : (k > kmax ? kmax : k));
 }
 inline unsigned march_blocks(const Geom &g, int kseg) {
   return (unsigned)(((g.nx + TXT - 1) / TXT) * ((g.ny + TYT - 1) / TYT) * ((g.nz + kseg - 1) / kseg));
@@ -569,7 +573,7 @@ inline bool all_periodic(const Block &B) {
 
 template <typename RT>
 hipError_t visc_flux_as(const Block &B, hipStream_t st) {
-  const int kseg = march_kseg(B.g);
+  const int kseg = march_kseg<RT>(B.g);
   const dim3 grid(march_blocks(B.g, kseg));
   const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
   if (cd8 && per) hipLaunchKernelGGL((k_visc_flux<true, true, RT>), grid, dim3(NT), 0, st, B, kseg);
@@ -587,7 +591,7 @@ hipError_t visc_div_as(const Block &B, hipStream_t st) {
 
 template <typename RT>
 hipError_t visc_div_update_as(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
-  const int kseg = march_kseg(B.g);
+  const int kseg = march_kseg<RT>(B.g);
   const dim3 grid(march_blocks(B.g, kseg));
   const bool cd8 = B.S.scheme_vis == SCH_CD8, per = all_periodic(B);
 #define OCFD_DIVUPD(V, C, P) \
