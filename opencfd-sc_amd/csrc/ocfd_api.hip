@@ -56,7 +56,10 @@ struct ocfd_handle {
   BLArgs bl = {};                  // boundary-layer / swept-corner data (ibc 2, 3)
   bool bl_set = false;
   std::vector<void *> bl_mem;      // its device arrays
-  hipGraphExec_t gexec = nullptr;  // one time step (three stages), captured on first use
+  // one time step (three stages), captured on first use; [p] starts with the
+  // temperature buffers in parity p (tpar), which every stage flips
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int tpar = 0;
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
@@ -387,6 +390,13 @@ int filtering(ocfd_handle *h) {
   return 0;
 }
 
+// after every stage's T <= 0 repair: this stage's temperature (T2) becomes
+// the current one (Block::T2)
+void swap_T(ocfd_handle *h) {
+  std::swap(h->B.T, h->B.T2);
+  h->tpar ^= 1;
+}
+
 int read_neg(ocfd_handle *h, int out[3]) {
   HIPCHK(hipMemcpyAsync(out, h->neg, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemcpyAsync(out + 1, h->neg + 65, 2 * sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -424,8 +434,9 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   int rc = 0;
   // the primitives are FP32 in the mixed-precision form
   const bool p32 = B.prec == OCFD_PREC_MIXED_CODE;
-  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) p->p = nullptr;
-  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) rc |= alloc_prim(p, B.g.nh, p32);
+  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) p->p = nullptr;
+  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) rc |= alloc_prim(p, B.g.nh, p32);
+  B.halt = nullptr;
   rc |= alloc(&B.f, fb) | alloc(&B.fn, fb) | alloc(&B.du, fb);
   if (cfg->scheme_invis == SCH_HYBRID) rc |= alloc(&B.rhyb, hb);
   if (cfg->if_viscous == 1) rc |= alloc(&B.ev, 12 * (size_t)B.g.ni * sizeof(double));
@@ -468,7 +479,7 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
     B.M.u[m] = 1.0;
   }
   // allocate_flow_data initial values (OCFD_NS_Solver.f90:116-118, 140-147)
-  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T}) (void)fill(*p, B.g.nh, 1.0, h->st);
+  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) (void)fill(*p, B.g.nh, 1.0, h->st);
   if (!B.M.uniform)
     for (int m = 0; m < M_N; m++) (void)fill(h->met[m], B.g.nh, 1.0, h->st);
   if (B.rhyb) (void)fill(B.rhyb, B.g.nh, 0.0, h->st);
@@ -503,7 +514,7 @@ int ocfd_destroy(ocfd_handle *h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   invalidate_graph(h);
   Block &B = h->B;
-  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T})
+  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2})
     if (p->p) (void)hipFree(p->p);
   double *ps[] = {B.f, B.fn, B.du, B.rhyb, B.ev, B.trash, h->fbuf, h->work, h->wout};
   for (double *p : ps)
@@ -801,6 +812,7 @@ int ocfd_comput_duvwt(ocfd_handle *h) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
   if (launch(h, "duvwt", [&] { return launch_duvwt(h->B, h->neg, h->st); })) return -1;
   if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+  swap_T(h);
   int c[3];
   if (read_neg(h, c)) return -1;
   return c[1];  // cells with T <= 0 found (and repaired) in this call
@@ -913,6 +925,7 @@ int step_body(ocfd_handle *h) {
     if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
       return -1;
     if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+    swap_T(h);
     if (h->ibc != 0) {
       if ((h->ibc == 2 || h->ibc == 3) && !h->bl_set)
         return fail("BC_BoundaryLayer / BC_SweptCorner: call ocfd_set_bc_boundarylayer first");
@@ -930,6 +943,7 @@ int step_body(ocfd_handle *h) {
 // host synchronisation happens inside it).
 int capture_step(ocfd_handle *h) {
   hipGraph_t graph = nullptr;
+  hipGraphExec_t *slot = &h->gexec[h->tpar];  // step_body flips tpar
   HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
   const int rc = step_body(h);
   const hipError_t e = hipStreamEndCapture(h->st, &graph);
@@ -937,10 +951,10 @@ int capture_step(ocfd_handle *h) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
-  const hipError_t ei = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+  const hipError_t ei = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) {
-    h->gexec = nullptr;
+    *slot = nullptr;
     return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
   return 0;
@@ -948,30 +962,64 @@ int capture_step(ocfd_handle *h) {
 }  // namespace
 
 void invalidate_graph(ocfd_handle *h) {
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  h->gexec = nullptr;
+  for (hipGraphExec_t &g : h->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 int ocfd_step(ocfd_handle *h, int32_t nsteps) {
   const ocfd_config &c = h->cfg;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, NEG_STRIDE);
   // filtering decides per step on the host; per-kernel timing needs eager launches
   const bool graph = h->use_graph && !h->prof && c.nfiltering == 0;
-  for (int n = 0; n < nsteps; n++) {
+  const int istep0 = h->istep;
+  const double tt0 = h->tt;
+  // the first stage with more than NEG_MAX cells T <= 0 halts the run on the
+  // device (Block::halt); the host looks every HALT_POLL steps and stops
+  constexpr int HALT_POLL = 8;
+  h->B.halt = h->neg + NEG_HALT;
+  int rc = 0;
+  for (int n = 0; n < nsteps && !rc; n++) {
     if (graph) {
-      if (!h->gexec && capture_step(h)) return -1;
-      HIPCHK(hipGraphLaunch(h->gexec, h->st));
-    } else if (step_body(h)) {
-      return -1;
+      const int p = h->tpar;
+      if (!h->gexec[p]) {
+        rc = capture_step(h);  // records the three stages, flipping tpar
+      } else {
+        swap_T(h);  // what the replayed stages do to the buffers
+      }
+      if (!rc && hipGraphLaunch(h->gexec[p], h->st) != hipSuccess) rc = fail("hipGraphLaunch failed");
+    } else {
+      rc = step_body(h);
     }
+    if (rc) break;
     h->istep += 1;
     h->tt = h->tt + c.dt;
-    if (filtering(h)) return -1;
+    rc = filtering(h);
+    if (!rc && (n + 1) % HALT_POLL == 0 && n + 1 < nsteps) {
+      int flag = 0;
+      if (hipMemcpyAsync(&flag, h->neg + NEG_HALT, sizeof(int), hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+          hipStreamSynchronize(h->st) != hipSuccess)
+        rc = fail("reading the T <= 0 counters failed");
+      if (flag) break;
+    }
   }
-  int cnt[3];
-  if (read_neg(h, cnt)) return -1;
-  if (cnt[2] > 10) return fail("more than 10 cells with T <= 0 in one stage");
-  return cnt[1];
+  h->B.halt = nullptr;
+  if (rc) return -1;
+  int cnt[NEG_STRIDE];
+  HIPCHK(hipMemcpyAsync(cnt, h->neg, sizeof cnt, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (cnt[NEG_HALT]) {
+    // the failing stage's update and repair stand; Istep / tt count the
+    // steps completed before it
+    const int stage = cnt[NEG_HALT_STAGE], whole = (stage - 1) / 3;
+    h->istep = istep0 + whole;
+    h->tt = tt0;
+    for (int q = 0; q < whole; q++) h->tt = h->tt + c.dt;
+    return fail("more than 10 cells with T <= 0 in stage " + std::to_string((stage - 1) % 3 + 1) + " of step " +
+                std::to_string(h->istep + 1) + " (" + std::to_string(cnt[66]) + " cells); the run stops there");
+  }
+  return cnt[65];
 }
 
 // ---- sweep tiles ---------------------------------------------------------------
@@ -1122,7 +1170,10 @@ struct ocfd_batch {
   hipStream_t st = nullptr;
   int prec = OCFD_PREC_FP64_CODE;
   std::vector<ocfd_handle *> box;
-  Block *dblk = nullptr;      // device copy of the boxes' Block descriptors
+  // device copies of the boxes' Block descriptors: [0] as the boxes hold
+  // them when a call starts, [1] with T and T2 swapped; stage s of step n
+  // reads table (n + s) % 2 (every stage swaps the temperature buffers)
+  Block *dblk[2] = {nullptr, nullptr};
   int *dtab = nullptr;        // box lists and tile offsets of every launch
   int *neg = nullptr;         // NEG_STRIDE ints per box
   std::vector<int> tab;       // host image of dtab
@@ -1146,15 +1197,16 @@ struct ocfd_batch {
   std::vector<SweepGroup> sweep;
   std::vector<ViscGroup> vis;
   Launch halo;
+  Launch hcopy;               // the T -> T2 halo copies after each stage update
   bool use_graph = true;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step starting from table [q]
   int64_t cells = 0;
 };
 
 namespace {
 
-BoxSet boxset(const ocfd_batch *b, const ocfd_batch::Launch &l) {
-  return BoxSet{b->dblk, b->dtab + l.ids, b->dtab + l.off, l.m};
+BoxSet boxset(const ocfd_batch *b, const ocfd_batch::Launch &l, int table = 0) {
+  return BoxSet{b->dblk[table], b->dtab + l.ids, b->dtab + l.off, l.m};
 }
 
 // box list + tile offsets of one launch, appended to the host table
@@ -1176,71 +1228,77 @@ ocfd_batch::Launch add_launch(ocfd_batch *b, const std::vector<int> &ids,
 }
 
 void batch_invalidate(ocfd_batch *b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  b->gexec = nullptr;
+  for (hipGraphExec_t &g : b->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 void batch_free(ocfd_batch *b) {
   batch_invalidate(b);
   for (ocfd_handle *h : b->box) ocfd_destroy(h);
-  if (b->dblk) (void)hipFree(b->dblk);
+  for (Block *t : b->dblk)
+    if (t) (void)hipFree(t);
   if (b->dtab) (void)hipFree(b->dtab);
   if (b->neg) (void)hipFree(b->neg);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
 
-int batch_sweep(ocfd_batch *b, const ocfd_batch::SweepGroup &g, int a, int shape) {
+int batch_sweep(ocfd_batch *b, const ocfd_batch::SweepGroup &g, int a, int shape, int table = 0) {
   static const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
   const ocfd_batch::Launch &l = g.l[a][shape];
-  const hipError_t e = launch_inviscid_batch(boxset(b, l), b->prec, a, g.sch, g.ch, modes[a], shape,
+  const hipError_t e = launch_inviscid_batch(boxset(b, l, table), b->prec, a, g.sch, g.ch, modes[a], shape,
                                              (int)l.ntiles, b->st);
   return e == hipSuccess ? 0 : fail(std::string("batched sweep: ") + hipGetErrorString(e));
 }
 
-int batch_halo(ocfd_batch *b) {
-  const hipError_t e = launch_halo_batch(boxset(b, b->halo), (int)b->halo.ntiles, b->st);
+int batch_halo(ocfd_batch *b, int table = 0) {
+  const hipError_t e = launch_halo_batch(boxset(b, b->halo, table), (int)b->halo.ntiles, b->st);
   return e == hipSuccess ? 0 : fail(std::string("batched halo fill: ") + hipGetErrorString(e));
 }
 
-// the three stages of one time step of every box (OCFD_NS_Solver.f90:33-60)
-int batch_step_body(ocfd_batch *b) {
+// the three stages of one time step of every box (OCFD_NS_Solver.f90:33-60),
+// the first reading descriptor table q0
+int batch_step_body(ocfd_batch *b, int q0) {
   for (int krk = 1; krk <= 3; krk++) {
-    if (batch_halo(b)) return -1;
+    const int q = q0 ^ ((krk - 1) & 1);
+    if (batch_halo(b, q)) return -1;
     for (const auto &g : b->sweep)
       for (int a = 0; a < 3; a++)
-        if (batch_sweep(b, g, a, g.shape[a])) return -1;
+        if (batch_sweep(b, g, a, g.shape[a], q)) return -1;
     for (const auto &v : b->vis) {
       if (!v.visc) continue;
       const auto &l = v.l[b->kq];
-      const hipError_t e = launch_visc_flux_batch(boxset(b, l), b->prec, v.cd8, (int)l.ntiles, b->st);
+      const hipError_t e = launch_visc_flux_batch(boxset(b, l, q), b->prec, v.cd8, (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched viscous flux: ") + hipGetErrorString(e));
     }
     for (const auto &v : b->vis) {
       const auto &l = v.l[b->kq];
-      const hipError_t e = launch_visc_div_update_batch(boxset(b, l), b->prec, v.visc, v.cd8, krk, b->neg,
+      const hipError_t e = launch_visc_div_update_batch(boxset(b, l, q), b->prec, v.visc, v.cd8, krk, b->neg,
                                                         (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched stage update: ") + hipGetErrorString(e));
     }
-    const hipError_t e = launch_negt_repair_batch(b->dblk, (int)b->box.size(), b->neg, b->st);
+    const hipError_t e = launch_negt_repair_batch(boxset(b, b->hcopy, q), (int)b->box.size(), b->neg,
+                                                  (int)b->hcopy.ntiles, b->st);
     if (e != hipSuccess) return fail(std::string("batched T <= 0 repair: ") + hipGetErrorString(e));
   }
   return 0;
 }
 
-int batch_capture(ocfd_batch *b) {
+int batch_capture(ocfd_batch *b, int q0) {
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  const int rc = batch_step_body(b);
+  const int rc = batch_step_body(b, q0);
   const hipError_t e = hipStreamEndCapture(b->st, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
-  const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  const hipError_t ei = hipGraphInstantiate(&b->gexec[q0], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) {
-    b->gexec = nullptr;
+    b->gexec[q0] = nullptr;
     return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
   return 0;
@@ -1250,13 +1308,19 @@ int batch_capture(ocfd_batch *b) {
 // own streams are drained first (set-up work may still run there)
 int batch_sync_tables(ocfd_batch *b) {
   HIPCHK(hipSetDevice(b->device));
-  std::vector<Block> blk;
-  for (ocfd_handle *h : b->box) {
+  std::vector<Block> blk[2];
+  for (size_t i = 0; i < b->box.size(); i++) {
+    ocfd_handle *h = b->box[i];
     HIPCHK(hipStreamSynchronize(h->st));
-    blk.push_back(h->B);
-    blk.back().kseg = BATCH_KSEGS[b->kq];
+    Block B = h->B;
+    B.kseg = BATCH_KSEGS[b->kq];
+    B.halt = b->neg + NEG_STRIDE * i + NEG_HALT;
+    blk[0].push_back(B);
+    std::swap(B.T, B.T2);
+    blk[1].push_back(B);
   }
-  HIPCHK(hipMemcpy(b->dblk, blk.data(), blk.size() * sizeof(Block), hipMemcpyHostToDevice));
+  for (int q = 0; q < 2; q++)
+    HIPCHK(hipMemcpy(b->dblk[q], blk[q].data(), blk[q].size() * sizeof(Block), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->dtab, b->tab.data(), b->tab.size() * sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
@@ -1319,6 +1383,7 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   std::vector<int> all(nbox);
   for (int i = 0; i < nbox; i++) all[i] = i;
   b->halo = add_launch(b, all, [](const Geom &g) { return halo_tiles(g); });
+  b->hcopy = add_launch(b, all, [](const Geom &g) { return halo_copy_tiles(g); });
   for (auto &g : b->sweep)
     for (int a = 0; a < 3; a++)
       for (int sh = 0; sh < NSWEEP_SHAPES; sh++)
@@ -1335,7 +1400,7 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   int rc = 0;
   rc |= hipSetDevice(device) != hipSuccess;
   rc |= hipStreamCreate(&b->st) != hipSuccess;
-  rc |= hipMalloc(&b->dblk, nbox * sizeof(Block)) != hipSuccess;
+  for (int q = 0; q < 2; q++) rc |= hipMalloc(&b->dblk[q], nbox * sizeof(Block)) != hipSuccess;
   rc |= hipMalloc(&b->dtab, b->tab.size() * sizeof(int)) != hipSuccess;
   rc |= hipMalloc(&b->neg, (size_t)NEG_STRIDE * nbox * sizeof(int)) != hipSuccess;
   if (rc) {
@@ -1363,27 +1428,62 @@ int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
   if (batch_sync_tables(b)) return -1;
   hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((NEG_STRIDE * nbox + 127) / 128)), dim3(128), 0, b->st, b->neg,
                      NEG_STRIDE * nbox);
-  for (int n = 0; n < nsteps; n++) {
-    if (b->use_graph) {
-      if (!b->gexec && batch_capture(b)) return -1;
-      HIPCHK(hipGraphLaunch(b->gexec, b->st));
-    } else if (batch_step_body(b)) {
-      return -1;
-    }
-    for (ocfd_handle *h : b->box) {
-      h->istep += 1;
-      h->tt = h->tt + h->cfg.dt;
-    }
+  std::vector<int> istep0(nbox);
+  std::vector<double> tt0(nbox);
+  for (int i = 0; i < nbox; i++) {
+    istep0[i] = b->box[i]->istep;
+    tt0[i] = b->box[i]->tt;
   }
   std::vector<int> neg((size_t)NEG_STRIDE * nbox);
-  HIPCHK(hipMemcpyAsync(neg.data(), b->neg, neg.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
-  HIPCHK(hipStreamSynchronize(b->st));
-  int total = 0;
-  for (int i = 0; i < nbox; i++) {
-    if (neg[(size_t)NEG_STRIDE * i + 66] > 10)
-      return fail("box " + std::to_string(i) + ": more than 10 cells with T <= 0 in one stage");
-    total += neg[(size_t)NEG_STRIDE * i + 65];
+  auto read_counters = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(neg.data(), b->neg, neg.size() * sizeof(int), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    return 0;
+  };
+  int n = 0, rc = 0;
+  for (; n < nsteps && !rc; n++) {
+    const int q0 = n & 1;
+    if (b->use_graph) {
+      if (!b->gexec[q0]) rc = batch_capture(b, q0);
+      if (!rc && hipGraphLaunch(b->gexec[q0], b->st) != hipSuccess) rc = fail("hipGraphLaunch failed");
+    } else {
+      rc = batch_step_body(b, q0);
+    }
+    // stop early once every box has stopped at too many T <= 0 cells
+    if (!rc && (n + 1) % 8 == 0 && n + 1 < nsteps) {
+      rc = read_counters();
+      bool all = true;
+      for (int i = 0; i < nbox && all; i++) all = neg[(size_t)NEG_STRIDE * i + NEG_HALT] != 0;
+      if (all) {
+        n++;
+        break;
+      }
+    }
   }
+  if (rc) return -1;
+  if (read_counters()) return -1;
+  // the boxes' temperature buffers swapped once per stage
+  for (ocfd_handle *h : b->box)
+    if ((3 * n) & 1) swap_T(h);
+  int total = 0;
+  std::string err;
+  for (int i = 0; i < nbox; i++) {
+    ocfd_handle *h = b->box[i];
+    const int *c = &neg[(size_t)NEG_STRIDE * i];
+    int whole = n;
+    if (c[NEG_HALT]) {
+      whole = (c[NEG_HALT_STAGE] - 1) / 3;
+      if (err.empty())
+        err = "box " + std::to_string(i) + ": more than 10 cells with T <= 0 in stage " +
+              std::to_string((c[NEG_HALT_STAGE] - 1) % 3 + 1) + " of step " + std::to_string(istep0[i] + whole + 1) +
+              "; that box stops there";
+    }
+    h->istep = istep0[i] + whole;
+    h->tt = tt0[i];
+    for (int q = 0; q < whole; q++) h->tt = h->tt + h->cfg.dt;
+    total += c[65];
+  }
+  if (!err.empty()) return fail(err);
   return total;
 }
 

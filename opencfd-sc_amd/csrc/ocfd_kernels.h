@@ -39,6 +39,14 @@ struct Block {
   Phys P;
   Metrics M;
   PrimField d, u, v, w, T;    // halo'd primitives
+  // The temperature the stage being computed writes.  comput_duvwT repairs a
+  // cell with T <= 0 from its six neighbours in the reference's i-j-k loop,
+  // so the -i/-j/-k neighbours hold this stage's T and the +i/+j/+k ones the
+  // previous stage's (OCFD_Time_Adv.f90:45-58, OCFD_handle_NegativeT.f90:11).
+  // The stage update therefore writes T2 and leaves T untouched; the repair
+  // reads both, copies T's halo into T2, and the host swaps the two after
+  // every stage (no extra HBM traffic per cell).
+  PrimField T2;
   double *f, *fn, *du;        // 5 planes each
   double *rhyb;               // halo'd sensor (hybrid scheme only)
   double *ev;                 // 12 interior planes: Ev1(1:4), Ev2(1:4), Ev3(1:4)
@@ -55,13 +63,21 @@ struct Block {
   double *wflux[3];
   int prec;  // OCFD_PREC_FP64_CODE / OCFD_PREC_MIXED_CODE (host-side dispatch)
   int kseg;  // boxes of a batch: planes per workgroup of the k-marching kernels
+  // Whole-step launches only (NULL otherwise): set by the repair of the first
+  // stage with more than 10 cells T <= 0 (handle_NegativeT stops the run
+  // there, OCFD_handle_NegativeT.f90:13-16).  From then on the stage update
+  // only carries T over, and the repair, boundary and filter kernels do
+  // nothing, so the state stays that of the failing stage.
+  int *halt;
 };
 // the kernels of a precision read the primitives in their real type: FP32
 // fields belong to the mixed-precision form, FP64 fields to the FP64 path
 inline bool prims_match(const Block &B) {
   const int f32 = B.prec == OCFD_PREC_MIXED_CODE;
-  return B.d.f32 == f32 && B.u.f32 == f32 && B.v.f32 == f32 && B.w.f32 == f32 && B.T.f32 == f32;
+  return B.d.f32 == f32 && B.u.f32 == f32 && B.v.f32 == f32 && B.w.f32 == f32 && B.T.f32 == f32 &&
+         B.T2.f32 == f32;
 }
+__device__ inline bool halted(const Block &B) { return B.halt != nullptr && *B.halt != 0; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2.  xcd_tile
@@ -76,8 +92,12 @@ __device__ inline int xcd_tile(int b, int nb) {
 }
 
 // ints per block of the T <= 0 bookkeeping: [0] cells found in the current
-// stage, [1..64] their interior indices, [65] total, [66] largest per stage
+// stage, [1..64] their interior indices, [65] total, [66] largest per stage,
+// [NEG_HALT] halt flag (Block::halt), [NEG_HALT_STAGE] the stage that set it,
+// [NEG_STAGES] stages completed since the counters were cleared
 constexpr int NEG_STRIDE = 72;
+constexpr int NEG_HALT = 67, NEG_HALT_STAGE = 68, NEG_STAGES = 69;
+constexpr int NEG_MAX = 10;  // Max_NegT (OCFD_handle_NegativeT.f90:6)
 
 // The boxes of one batched launch (ocfd_batch): workgroup tiles of box
 // ids[j] are off[j] .. off[j+1]-1 of the launch.  box() finds the box of a
@@ -191,7 +211,11 @@ hipError_t launch_visc_flux_batch(const BoxSet &S, int prec, bool cd8, int ntile
 hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bool cd8, int krk, int *neg,
                                         int ntiles, hipStream_t st);
 hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st);
-hipError_t launch_negt_repair_batch(const Block *blk, int nbox, int *neg, hipStream_t st);
+// the T <= 0 repair of every box (workgroup b repairs box b of the table) and
+// the copy of T's halo into T2, over the halo-copy tiles of S
+hipError_t launch_negt_repair_batch(const BoxSet &S, int nbox, int *neg, int ntiles, hipStream_t st);
+// workgroups of the T -> T2 halo copy of one box
+int64_t halo_copy_tiles(const Geom &g);
 // workgroups of the k-marching kernels of a block marched in segments of
 // kseg planes (a batch fills the chip with many boxes; its autotuner picks
 // the segment length from BATCH_KSEGS)
@@ -213,6 +237,8 @@ hipError_t launch_visc_div_update(const Block &B, int krk, int refresh_planes, i
 hipError_t launch_rk3(const Block &B, int krk, hipStream_t st);
 hipError_t launch_copy_f(const Block &B, hipStream_t st);
 hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st);
+// T <= 0 repair of the cells listed in neg (T2 from T2 / T neighbours) and
+// the copy of T's halo into T2, in one launch
 hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st);
 hipError_t launch_bc(const Block &B, int ibc, double tw, const BLArgs *bl, const int *ghost, int planes,
                      hipStream_t st);

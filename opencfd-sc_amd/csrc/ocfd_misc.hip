@@ -247,17 +247,31 @@ __global__ __launch_bounds__(NT) void k_duvwt(const Block B, int *neg) {
   const double f1 = B.f[t], f2 = B.f[t + ni], f3 = B.f[t + 2 * ni], f4 = B.f[t + 3 * ni], f5 = B.f[t + 4 * ni];
   const double u = f2 / f1, v = f3 / f1, w = f4 / f1;
   const double T = (f5 - (f2 * u + f3 * v + f4 * w) * 0.5) / (f1 * B.P.cv);
-  B.d[h] = f1; B.u[h] = u; B.v[h] = v; B.w[h] = w; B.T[h] = T;
+  // T goes to T2: the repair reads the previous stage's T of the +i/+j/+k
+  // neighbours from B.T
+  B.d[h] = f1; B.u[h] = u; B.v[h] = v; B.w[h] = w; B.T2[h] = T;
   if (T <= 0) {
     const int slot = atomicAdd(neg, 1);
     if (slot < 64) neg[1 + slot] = (int)t;
   }
 }
 
-// T <= 0 repair (OCFD_handle_NegativeT.f90:11): T = mean of the six
-// neighbours; the listed cells are processed in ascending index order by one
-// thread.  Neighbour values are those after the whole primitive update.
+// T <= 0 repair (handle_NegativeT, OCFD_handle_NegativeT.f90:11, called from
+// the i-j-k loop of comput_duvwT, OCFD_Time_Adv.f90:45-58): T = mean of the
+// six neighbours.  The listed cells are processed in ascending index order,
+// the reference's loop order, by one thread.  A -i/-j/-k neighbour inside the
+// block already holds this stage's value (repaired, if it was listed), read
+// from T2; a +i/+j/+k neighbour still holds the previous stage's, read from T;
+// so does every halo point (comput_duvwT writes none), also read from T
+// because the copy of T's halo into T2 runs concurrently.  With `halt` the
+// first stage with more than NEG_MAX cells sets the flag and records itself
+// (later stages find it set and repair nothing).
 __device__ void negt_repair_body(const Block &B, int *neg, const int refresh) {
+  if (halted(B)) {
+    neg[0] = 0;
+    return;
+  }
+  neg[NEG_STAGES] += 1;
   int cnt = neg[0];
   if (cnt <= 0) return;
   if (cnt > 64) cnt = 64;
@@ -272,29 +286,74 @@ __device__ void negt_repair_body(const Block &B, int *neg, const int refresh) {
     int i, j, k;
     cell_of(g, lst[a], i, j, k);
     const int64_t h = g.hi(i, j, k);
-    B.T[h] = (B.T[g.hi(i + 1, j, k)] + B.T[g.hi(i - 1, j, k)] + B.T[g.hi(i, j + 1, k)] + B.T[g.hi(i, j - 1, k)] +
-              B.T[g.hi(i, j, k + 1)] + B.T[g.hi(i, j, k - 1)]) / 6.0;
+    const double tip = B.T[g.hi(i + 1, j, k)], tjp = B.T[g.hi(i, j + 1, k)], tkp = B.T[g.hi(i, j, k + 1)];
+    const double tim = i > 1 ? double(B.T2[g.hi(i - 1, j, k)]) : double(B.T[g.hi(i - 1, j, k)]);
+    const double tjm = j > 1 ? double(B.T2[g.hi(i, j - 1, k)]) : double(B.T[g.hi(i, j - 1, k)]);
+    const double tkm = k > 1 ? double(B.T2[g.hi(i, j, k - 1)]) : double(B.T[g.hi(i, j, k - 1)]);
+    // the reference's summation order: i+1, i-1, j+1, j-1, k+1, k-1
+    B.T2[h] = (tip + tim + tjp + tjm + tkp + tkm) / 6.0;
     if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz)) {
       const int64_t ni = g.ni, q = lst[a];
       const double d = B.d[h], u = B.u[h], v = B.v[h], w = B.w[h];
-      B.f[q + 4 * ni] = d * (B.P.cv * B.T[h] + (u * u + v * v + w * w) * 0.5);
+      B.f[q + 4 * ni] = d * (B.P.cv * double(B.T2[h]) + (u * u + v * v + w * w) * 0.5);
     }
   }
   const int total = neg[0];
   neg[65] += total;
   if (total > neg[66]) neg[66] = total;
+  if (total > NEG_MAX && B.halt) {
+    *B.halt = 1;
+    neg[NEG_HALT_STAGE] = neg[NEG_STAGES];
+  }
   neg[0] = 0;
 }
 
-__global__ void k_negt_repair(const Block B, int *neg, const int refresh) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  negt_repair_body(B, neg, refresh);
+// Copy of T's halo points into T2 (the stage update and comput_duvwT write
+// the interior of T2 only).  Tile q of a box covers HCOPY_ROWS (j, k) rows
+// of the halo'd array: rows outside the interior j, k range are copied whole,
+// the others at their 2 * LAP i-halo points.
+constexpr int HCOPY_ROWS = 16;
+__device__ inline void halo_copy_tile(const Block &B, const int64_t q) {
+  const Geom &g = B.g;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nrows = (int64_t)g.sy * g.sz;
+  const bool f32 = B.T.f32 != 0;
+  for (int r = wave; r < HCOPY_ROWS; r += NT / 64) {
+    const int64_t row = q * HCOPY_ROWS + r;
+    if (row >= nrows) break;
+    const int jh = (int)(row % g.sy), kh = (int)(row / g.sy);  // 0-based halo'd indices
+    const int64_t base = row * g.sx;
+    const bool whole = jh < LAP || jh >= LAP + g.ny || kh < LAP || kh >= LAP + g.nz;
+    const int npts = whole ? g.sx : 2 * LAP;
+    for (int p = lane; p < npts; p += 64) {
+      const int ih = whole ? p : (p < LAP ? p : g.nx + p);
+      const int64_t x = base + ih;
+      if (f32) B.T2.as<float>()[x] = B.T.as<float>()[x];
+      else B.T2.as<double>()[x] = B.T.as<double>()[x];
+    }
+  }
+}
+inline int64_t halo_copy_tiles_of(const Geom &g) { return ((int64_t)g.sy * g.sz + HCOPY_ROWS - 1) / HCOPY_ROWS; }
+
+// workgroup 0: the repair; workgroups 1..: the halo copy
+__global__ __launch_bounds__(NT) void k_negt_repair(const Block B, int *neg, const int refresh) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) negt_repair_body(B, neg, refresh);
+    return;
+  }
+  halo_copy_tile(B, blockIdx.x - 1);
 }
 
-// one workgroup per box of a batch
-__global__ void k_negt_repair_batch(const Block *blk, int *neg) {
-  if (threadIdx.x != 0) return;
-  negt_repair_body(blk[blockIdx.x], neg + NEG_STRIDE * blockIdx.x, 1);
+// batches: workgroup b < nbox repairs box b (its counters at neg + NEG_STRIDE
+// * b); the others copy halos over the tiles of S
+__global__ __launch_bounds__(NT) void k_negt_repair_batch(const BoxSet S, const int nbox, int *neg) {
+  if ((int)blockIdx.x < nbox) {
+    if (threadIdx.x == 0) negt_repair_body(S.blk[blockIdx.x], neg + NEG_STRIDE * blockIdx.x, 1);
+    return;
+  }
+  const int t = blockIdx.x - nbox;
+  const int j = S.entry(t);
+  halo_copy_tile(S.blk[S.ids[j]], t - S.off[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,6 +362,7 @@ __global__ void k_negt_repair_batch(const Block *blk, int *neg) {
 // BC hooks on the j=1 / j=ny planes: ocfd_bc_blunt2d (OCFD_BC_Def.f90:4-33),
 // or the isothermal/adiabatic wall + free stream of the BC_USER_DEF hook.
 __global__ __launch_bounds__(NT) void k_bc_hook(const Block B, const int ibc, const double Tw) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.nx * g.nz) return;
@@ -335,6 +395,7 @@ __device__ inline void set_prims(const Block &B, int64_t h, double d, double u, 
 
 // i = 1: free stream, or the inlet data
 __global__ __launch_bounds__(NT) void k_bl_inlet(const Block B, const BLArgs a) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.ny * g.nz) return;
@@ -349,6 +410,7 @@ __global__ __launch_bounds__(NT) void k_bl_inlet(const Block B, const BLArgs a) 
 
 // j = ny: Dirichlet values where the point is not non-reflecting
 __global__ __launch_bounds__(NT) void k_bl_upper(const Block B, const BLArgs a) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.nx * g.nz || a.nonref[t] != 0) return;
@@ -365,6 +427,7 @@ __global__ __launch_bounds__(NT) void k_bl_upper(const Block B, const BLArgs a) 
 // adiabatic wall with second-order pressure; swept corner: points off the
 // wall extrapolate at first order
 __global__ __launch_bounds__(NT) void k_bl_wall(const Block B, const BLArgs a) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.nx * g.nz) return;
@@ -391,6 +454,7 @@ __global__ __launch_bounds__(NT) void k_bl_wall(const Block B, const BLArgs a) {
 
 // i = nx: first- or second-order extrapolation
 __global__ __launch_bounds__(NT) void k_bl_outlet(const Block B, const BLArgs a) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.ny * g.nz) return;
@@ -402,6 +466,7 @@ __global__ __launch_bounds__(NT) void k_bl_outlet(const Block B, const BLArgs a)
 
 // swept corner, k = 1 / k = nz: first-order extrapolation where requested
 __global__ __launch_bounds__(NT) void k_bl_sides(const Block B, const BLArgs a) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)g.nx * g.ny) return;
@@ -415,6 +480,7 @@ __global__ __launch_bounds__(NT) void k_bl_sides(const Block B, const BLArgs a) 
 
 // f recomputed from the primitives on the six block faces (OCFD_BC.f90:132-200)
 __global__ __launch_bounds__(NT) void k_bc_planes(const Block B) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t ax = (int64_t)g.ny * g.nz, ay = (int64_t)g.nx * g.nz, az = (int64_t)g.nx * g.ny;
   int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -443,6 +509,7 @@ __global__ __launch_bounds__(NT) void k_bc_planes(const Block B) {
 
 // Flow_Ghost_Extent_2nd (OCFD_GhostCell.f90:164-286)
 __global__ __launch_bounds__(NT) void k_ghost(const Block B, const int side) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int axis = side / 2, hi = side % 2;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;  // the two other axes
@@ -479,6 +546,7 @@ __global__ __launch_bounds__(NT) void k_ghost(const Block B, const int side) {
 // Flow_Ghost_flatplate :110-136) or swept-corner symmetry plane k- (w
 // changes sign, Flow_Ghost_Sweptcorner :56-76)
 __global__ __launch_bounds__(NT) void k_ghost_user(const Block B, const int ibc) {
+  if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (ibc == 2) {
@@ -533,8 +601,9 @@ __global__ __launch_bounds__(NT) void k_filter(const Block B, const FilterArgs A
   const int axis = A.axis, n = g.n(axis);
   const int s = idx[axis];
   const int64_t st = g.istride(axis), ni = g.ni;
-  const bool in = idx[0] >= A.lo[0] && idx[0] <= A.hi[0] && idx[1] >= A.lo[1] && idx[1] <= A.hi[1] &&
-                  idx[2] >= A.lo[2] && idx[2] <= A.hi[2];
+  // a whole-step run stopped at too many T <= 0 cells carries f over unchanged
+  const bool in = !halted(B) && idx[0] >= A.lo[0] && idx[0] <= A.hi[0] && idx[1] >= A.lo[1] &&
+                  idx[1] <= A.hi[1] && idx[2] >= A.lo[2] && idx[2] <= A.hi[2];
   auto wrap = [&](int q) { return q < 1 ? q + n : (q > n ? q - n : q); };
   for (int m = 0; m < 5; m++) {
     const double *f0 = B.f + (int64_t)m * ni;
@@ -949,13 +1018,15 @@ hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_negt_repair_batch(const Block *blk, int nbox, int *neg, hipStream_t st) {
-  hipLaunchKernelGGL(k_negt_repair_batch, dim3((unsigned)nbox), dim3(64), 0, st, blk, neg);
+int64_t halo_copy_tiles(const Geom &g) { return halo_copy_tiles_of(g); }
+
+hipError_t launch_negt_repair_batch(const BoxSet &S, int nbox, int *neg, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_negt_repair_batch, dim3((unsigned)(nbox + ntiles)), dim3(NT), 0, st, S, nbox, neg);
   return hipGetLastError();
 }
 
 hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st) {
-  hipLaunchKernelGGL(k_negt_repair, dim3(1), dim3(64), 0, st, B, neg, 1);
+  hipLaunchKernelGGL(k_negt_repair, dim3((unsigned)(1 + halo_copy_tiles_of(B.g))), dim3(NT), 0, st, B, neg, 1);
   return hipGetLastError();
 }
 

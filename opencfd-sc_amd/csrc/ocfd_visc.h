@@ -338,6 +338,16 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
   auto kwrap = [&](int t) {
     return perz ? (t < 1 ? t + nz : (t > nz ? t - nz : t)) : (t < 1 ? 1 : (t > nz ? nz : t));
   };
+  if (halted(B)) {
+    // a whole-step run stopped at too many T <= 0 cells: the state stays, and
+    // T carries over into the buffer the host swaps in after the stage
+    if (tc.valid)
+      for (int k = tc.kb; k <= tc.ke; k++) {
+        const int64_t h = g.hi(ic, jc, k);
+        B.T2.as<RT>()[h] = B.T.as<RT>()[h];
+      }
+    return;
+  }
 
   // in-plane offsets of this lane's shares of the Ev1 / Ev2 tiles
   int o1[NR1], o2[NR2];
@@ -486,7 +496,9 @@ __device__ __forceinline__ void visc_div_update_tile(const Block &B, const int k
       __builtin_nontemporal_store(RT(u), tc.valid ? B.u.as<RT>() + ph : pt + NT);
       __builtin_nontemporal_store(RT(v), tc.valid ? B.v.as<RT>() + ph : pt + 2 * NT);
       __builtin_nontemporal_store(RT(w), tc.valid ? B.w.as<RT>() + ph : pt + 3 * NT);
-      __builtin_nontemporal_store(RT(T), tc.valid ? B.T.as<RT>() + ph : pt + 4 * NT);
+      // T into T2: the T <= 0 repair reads this stage's and the previous
+      // stage's values (Block::T2)
+      __builtin_nontemporal_store(RT(T), tc.valid ? B.T2.as<RT>() + ph : pt + 4 * NT);
     }
     negmask |= (tc.valid && T <= 0) ? (1ull << (k - tc.kb)) : 0ull;
     if (refresh && (i == 1 || i == g.nx || j == 1 || j == g.ny || k == 1 || k == g.nz))

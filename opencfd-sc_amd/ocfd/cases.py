@@ -347,6 +347,30 @@ def random_case(shape=(10, 12, 16), seed=1, scheme="WENO5", split=1, char=0, vis
     return case
 
 
+def negative_t_case(shape=(8, 10, 14), seed=44, amp=4.0, t_factor=0.1, dt_factor=4.0,
+                    centres=((2, 3, 4), (5, 6, 9), (6, 2, 11)), scheme="WENO5", split=1, char=0) -> Case:
+    """A seeded periodic test case (random_case) with strong local rarefactions:
+    at each centre (k, j, i) u drops by `amp` at i-1, i-2 and rises by `amp`
+    at i+1, i+2, and T is scaled by `t_factor`; dt is `dt_factor` times
+    random_case's.  The RK stages then produce a few cells with T <= 0 in
+    the fluid (not planted in f), whose neighbours also change within the
+    stage, which is what handle_NegativeT's treatment of the +i/+j/+k
+    neighbours depends on (OCFD_handle_NegativeT.f90:11).  With the defaults
+    every stage of the first three steps repairs 3 to 6 cells; amp=6,
+    t_factor=1, dt_factor=5 exceeds 10 cells in the first stage of step 2."""
+    case = random_case(shape=shape, seed=seed, scheme=scheme, split=split, char=char)
+    d, u, v, w, T = case.prims
+    nx = shape[2]
+    for (k, j, i) in centres:
+        for s in (1, 2):
+            u[k, j, (i - s) % nx] -= amp
+            u[k, j, (i + s) % nx] += amp
+        T[k, j, i] *= t_factor
+    case.cfg.dt *= dt_factor
+    case.name = "negt%d" % seed
+    return case
+
+
 def blunt_ogrid(nx=512, ny=256, nz=8, r0=1.0, dr0=None, ratio=1.05, zspan=0.1, ma=6.0,
                 re=1.0e5, t_inf=79.0, t_wall=None, cfl=0.5, filter_every=1, filter_s0=0.2) -> Case:
     """Mach-6 flow over a cylinder: O-grid (i around the body, periodic; j

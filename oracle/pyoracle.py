@@ -129,15 +129,20 @@ def reference_available():
     return os.path.exists(PROBE) and os.path.exists(REF_EXE) and os.path.exists(MPIEXEC)
 
 
-def run_probe(case, workdir, nextra=0, timeout=600):
-    """Run the probe driver on `case` in `workdir`; returns dict of arrays."""
+def run_probe(case, workdir, nextra=0, timeout=600, allow_negative_t=False):
+    """Run the probe driver on `case` in `workdir`; returns dict of arrays.
+    With allow_negative_t the cells the reference reports as "Negative T
+    found" (handle_NegativeT, OCFD_handle_NegativeT.f90:9, 1-based global
+    i, j, k in the order repaired) come back as res["negt_cells"]; a run the
+    reference stops (more than 10 in one stage) is an error either way."""
     os.makedirs(workdir, exist_ok=True)
     case.write_reference_inputs(workdir, nsteps=1 + nextra)
     out = os.path.join(workdir, "probe_out")
     os.makedirs(out, exist_ok=True)
     r = subprocess.run([MPIEXEC, "-n", "1", PROBE, out, str(nextra)], cwd=workdir,
                        capture_output=True, text=True, timeout=timeout)
-    if r.returncode != 0 or "Negative T" in r.stdout or "STOP" in r.stdout:
+    if r.returncode != 0 or "STOP" in r.stdout or "Limit" in r.stdout or \
+            ("Negative T" in r.stdout and not allow_negative_t):
         raise OcfdError("probe failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))
     cfg = case.cfg
     L = cfg.lap
@@ -159,6 +164,10 @@ def run_probe(case, workdir, nextra=0, timeout=600):
     if os.path.exists(hist):
         res["hist"] = np.loadtxt(hist, ndmin=2)
     res["stdout"] = r.stdout
+    if allow_negative_t:
+        cells = [[int(x) for x in line.split("!")[1].split()] for line in r.stdout.splitlines()
+                 if "Negative T found" in line]
+        res["negt_cells"] = np.array(cells, dtype=np.int32).reshape(-1, 3)
     return res
 
 

@@ -13,6 +13,9 @@ reference's outputs:
   rhyb      Rhybrid (interior) after comput_Rhybrid (hybrid cases)
   met       Ajac, Akx1, Aiy1, Asz1 (halo'd) after init_mesh (curvilinear cases)
   hist      per step: Istep, tt, E, K, S (show_flow_msg), KE, enstrophy (ana_box)
+  k1_T, n1_T, nN_T, negt_cells   (negt_* cases) T after stage 1, one and three
+            steps, and the cells handle_NegativeT repaired (global i, j, k in
+            the order the reference reported them)
 
 Also writes ref_snapshot_w5_sw.dat: the snapshot file (save_flow_data,
 OCFD_IO.f90:44-64) that the unmodified reference writes after two steps of the
@@ -72,6 +75,8 @@ def golden_cases():
                              bc="BC_BLUNT2D", mass_force=(0.1, -0.2, 0.05)),
         "hyb_sensor2_curv": C(seed=16, scheme="HYBRID", split=2, char=1, jump=0.6, grid="curv",
                               iperiodic=(1, 1, 0), hybrid=(1.0, 3.0), sensor=2),
+        # strong rarefactions: 3 to 6 cells with T <= 0 repaired in every stage
+        "negt_w5_sw": cases.negative_t_case(),
     }
 
 
@@ -116,7 +121,8 @@ def main():
         for name, case in golden_cases().items():
             if only and name not in only:
                 continue
-            ref = pyoracle.run_probe(case, os.path.join(tmp, name), nextra=2)
+            negt = name.startswith("negt")
+            ref = pyoracle.run_probe(case, os.path.join(tmp, name), nextra=2, allow_negative_t=negt)
             L = case.cfg.lap
             out = {
                 "config": np.array(cfg_to_json(case.cfg)),
@@ -125,6 +131,11 @@ def main():
                 "k1_duinv": ref["k1_duinv"], "k1_dutot": ref["k1_dutot"], "k1_f": ref["k1_f"],
                 "n1_f": ref["n1_f"], "nN_f": ref["nN_f"], "hist": ref["hist"],
             }
+            if negt:
+                # the temperature after stage 1, after one step and after three
+                # steps (halo'd), and the cells the reference repaired, in order
+                for k in ("k1_T", "n1_T", "nN_T", "negt_cells"):
+                    out[k] = ref[k]
             if "rhyb" in ref:
                 out["rhyb"] = ref["rhyb"][L:-L, L:-L, L:-L]
             if case.cfg.grid_type != cases.GRID_BOX:

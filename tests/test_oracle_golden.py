@@ -37,15 +37,24 @@ def test_oracle_stage1_and_steps_bitwise(name):
     np.testing.assert_array_equal(ob.download_du(), ref["k1_dutot"])
     ob.time_adv_rk3(1)
     np.testing.assert_array_equal(ob.download_f(), ref["k1_f"])
-    ob.comput_duvwt()
+    nneg = [ob.comput_duvwt()]
     ob.bc()
+    if "k1_T" in ref:
+        # T <= 0 cases: the repaired temperature, halos included, and the
+        # number of cells the reference reported
+        np.testing.assert_array_equal(ob.download_primitives()[4], ref["k1_T"])
     for krk in (2, 3):
-        ob.stage(krk)
+        nneg.append(ob.stage(krk))
     ob.advance_clock()
     ob.filtering()
     np.testing.assert_array_equal(ob.download_f(), ref["n1_f"])
-    ob.step(2)
+    nneg.append(ob.step(2))
     np.testing.assert_array_equal(ob.download_f(), ref["nN_f"])
+    if "k1_T" in ref:
+        np.testing.assert_array_equal(ob.download_primitives()[4], ref["nN_T"])
+        assert sum(nneg) == len(ref["negt_cells"]) and nneg[0] >= 2
+    else:
+        assert sum(nneg) == 0
 
 
 @pytest.mark.parametrize("name", ["w5_sw", "hyb_c_llf", "curv_wall_fo9p"])
