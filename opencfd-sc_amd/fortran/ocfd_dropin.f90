@@ -18,7 +18,17 @@
 !   OCFD_time_adv_RK3(KRK)       -> ocfd_save_fn (KRK=1: the fn=f loop of
 !                                   OCFD_NS_Solver.f90:23-31) + ocfd_time_adv_rk3
 !   comput_duvwT                 -> ocfd_comput_duvwt        (OCFD_Time_Adv.f90:40-63)
-!   OCFD_bc                      -> ocfd_bc                  (OCFD_BC.f90:2-99)
+!   OCFD_bc                      -> ocfd_bc                  (OCFD_BC.f90:2-99); with
+!                                   BoundaryCondition = BC_User_Def the
+!                                   user's own ocfd_bc_User runs on the host
+!                                   (OCFD_BC.f90:18-19): the state is copied
+!                                   down, the reference's OCFD_bc runs, the
+!                                   state is copied back.  OCFD_HIP_USER_BC=device
+!                                   instead runs the library's built-in hook on
+!                                   the device (isothermal or adiabatic no-slip
+!                                   wall at j=1, free stream at j=ny); only a
+!                                   case whose ocfd_bc_User is that wall may ask
+!                                   for it.
 !   filtering(f)                 -> ocfd_advance_clock + ocfd_filtering (OCFD_filtering.f90:4-70)
 module ocfd_dropin_state
    use iso_c_binding
@@ -27,6 +37,9 @@ module ocfd_dropin_state
    ! .true. while host code that reads the state runs (its own halo
    ! exchanges must then act on the host arrays)
    logical, save :: host_section = .false.
+   ! BC_User_Def served by the library's device hook (OCFD_HIP_USER_BC=device)
+   ! instead of the user's host routine
+   logical, save :: user_bc_on_device = .false.
 end module ocfd_dropin_state
 
 ! Creates the device block from the reference's parameters and uploads the
@@ -74,6 +87,16 @@ subroutine ocfd_dropin_attach()
    dev = 0
    call get_environment_variable('OCFD_HIP_DEVICE', env, status=st)
    if (st == 0) read (env, *) dev
+   user_bc_on_device = .false.
+   call get_environment_variable('OCFD_HIP_USER_BC', env, status=st)
+   if (st == 0) then
+      if (trim(env) == 'device') then
+         user_bc_on_device = .true.
+      else if (trim(env) /= 'host') then
+         print *, 'libocfd_hip drop-in: OCFD_HIP_USER_BC must be host or device, not ', trim(env)
+         stop
+      end if
+   end if
    call ocfd_check(ocfd_create(cfg, int(dev, c_int), hdev), 'ocfd_create')
 
    mets = [ocfd_addr(Akx), ocfd_addr(Aky), ocfd_addr(Akz), ocfd_addr(Aix), ocfd_addr(Aiy), &
@@ -86,6 +109,13 @@ subroutine ocfd_dropin_attach()
    call ocfd_check(ocfd_upload_f(hdev, f), 'ocfd_upload_f')
    if (Para%IBC == BC_BoundaryLayer .or. Para%IBC == BC_SweptCorner) call ocfd_dropin_bc_data()
    if (my_id .eq. 0) print *, 'libocfd_hip: block on GPU', dev
+   if (Para%IBC == BC_User_Def .and. my_id .eq. 0) then
+      if (user_bc_on_device) then
+         print *, 'libocfd_hip: BC_User_Def by the library''s device wall hook'
+      else
+         print *, 'libocfd_hip: BC_User_Def by the host routine ocfd_bc_User'
+      end if
+   end if
 end subroutine ocfd_dropin_attach
 
 ! The data of ocfd_bc_BoundaryLayer / ocfd_bc_swept_corner that the first
@@ -172,6 +202,17 @@ subroutine ocfd_dropin_download()
    call ocfd_check(ocfd_download_f(hdev, f), 'ocfd_download_f')
 end subroutine ocfd_dropin_download
 
+! Host arrays d,u,v,w,T (halo'd) and f -> device state.
+subroutine ocfd_dropin_upload()
+   use flow_data
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   call ocfd_check(ocfd_upload_primitives(hdev, d, u, v, w, T, int(Istep, c_int32_t), tt), &
+                   'ocfd_upload_primitives')
+   call ocfd_check(ocfd_upload_f(hdev, f), 'ocfd_upload_f')
+end subroutine ocfd_dropin_upload
+
 subroutine wrap_ocfd_bc() bind(C, name="__wrap_ocfd_bc_")
    use flow_data
    use ocfd_hip
@@ -187,6 +228,14 @@ subroutine wrap_ocfd_bc() bind(C, name="__wrap_ocfd_bc_")
       call ocfd_dropin_attach()
    else if (host_section) then
       call real_ocfd_bc()
+   else if (Para%IBC == BC_User_Def .and. .not. user_bc_on_device) then
+      ! the user's routine reads and writes host arrays: the whole OCFD_bc
+      ! (hook, boundary-plane f, ghost cells) runs on a copy of the state
+      call ocfd_dropin_download()
+      host_section = .true.
+      call real_ocfd_bc()
+      host_section = .false.
+      call ocfd_dropin_upload()
    else
       if (Para%IBC == BC_BoundaryLayer .or. Para%IBC == BC_SweptCorner) call ocfd_dropin_bc_time()
       call ocfd_check(ocfd_bc(hdev), 'ocfd_bc')

@@ -23,6 +23,8 @@ COUNT_LIB_PATH = os.path.join(HERE, "liboracle_count.so")
 REF_DIR = os.path.join(HERE, "_ref")
 PROBE = os.path.join(REF_DIR, "ocfd_probe")
 REF_EXE = os.path.join(REF_DIR, "opencfd_ref")
+# the reference with its empty ocfd_bc_User replaced by ref_user_hooks.f90
+REF_WALL_EXE = os.path.join(REF_DIR, "opencfd_ref_wall")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 _lib = None

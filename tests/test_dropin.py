@@ -14,6 +14,8 @@ from util import gpu_available
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DROPIN = os.path.join(ROOT, "opencfd-sc_amd", "fortran", "_build", "opencfd_hip")
+# the same with the wall hook of oracle/ref_user_hooks.f90 as the user's ocfd_bc_User
+DROPIN_WALL = os.path.join(ROOT, "opencfd-sc_amd", "fortran", "_build", "opencfd_hip_wall")
 FC_MOD = os.path.join(ROOT, "opencfd-sc_amd", "fortran", "ocfd_hip_mod.f90")
 
 # (case name, steps): periodic hybrid, curvilinear closures, filters,
@@ -38,12 +40,13 @@ def _case(name):
     return golden_cases()[name]
 
 
-def _run(exe, workdir, case, nsteps):
+def _run(exe, workdir, case, nsteps, env=None):
     from pyoracle import MPIEXEC
     os.makedirs(workdir, exist_ok=True)
     case.cfg.istep_show = 1
     case.write_reference_inputs(workdir, nsteps=nsteps, ana_box_every=1, istep_save=1)
-    r = subprocess.run([MPIEXEC, "-n", "1", exe], cwd=workdir, capture_output=True, text=True, timeout=300)
+    r = subprocess.run([MPIEXEC, "-n", "1", exe], cwd=workdir, capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, **(env or {})))
     out = r.stdout + r.stderr
     assert r.returncode == 0, out[-3000:]
     assert "STOP" not in out and "failed" not in out, out[-3000:]
@@ -85,6 +88,53 @@ def test_dropin_matches_reference_files(tmp_path, name, nsteps):
         assert a == b, fn
     assert _sums(out_ref) == _sums(out_gpu)
     assert len(_sums(out_ref)) >= nsteps
+
+
+def _compare_files(tmp_path, ref, gpu, nsteps):
+    files = sorted(os.path.basename(p) for p in glob.glob(str(tmp_path / ref / "OCFD[0-9]*.dat")))
+    assert len(files) == nsteps
+    for fn in files + ["statistics.dat"]:
+        a = open(tmp_path / ref / fn, "rb").read()
+        b = open(tmp_path / gpu / fn, "rb").read()
+        assert a == b, fn
+
+
+def _small_blunt():
+    from ocfd import cases
+    return cases.blunt_ogrid(nx=48, ny=24, nz=8)
+
+
+# BoundaryCondition = BC_User_Def: the user's ocfd_bc_User runs on the host by
+# default (OCFD_BC.f90:18-19).  With the reference's own, empty, hook the
+# drop-in must reproduce the reference (no wall appears); with the wall hook
+# of ref_user_hooks.f90 linked into both executables, the host pass-through
+# and the library's device hook (OCFD_HIP_USER_BC=device) must both
+# reproduce the reference built with that hook.
+USER_BC = [("curv_sb_ghost", "stub", "host"), ("curv_wall_fo9p", "wall", "host"),
+           ("curv_wall_fo9p", "wall", "device"), ("blunt_small", "wall", "host"),
+           ("blunt_small", "wall", "device")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,hook,mode", USER_BC)
+def test_dropin_user_bc(tmp_path, name, hook, mode):
+    import pyoracle
+    if not gpu_available():
+        pytest.skip("no GPU")
+    exe, ref = (DROPIN, pyoracle.REF_EXE) if hook == "stub" else (DROPIN_WALL, pyoracle.REF_WALL_EXE)
+    for p in (exe, ref):
+        if not os.path.exists(p):
+            pytest.skip("%s not built" % p)
+    case = _small_blunt() if name == "blunt_small" else _case(name)
+    assert case.cfg.boundary_condition.upper() == "BC_USER_DEF"
+    print("drop-in user BC", name, hook, mode, case.cfg.shape, flush=True)
+    nsteps = 3
+    out_ref = _run(ref, str(tmp_path / "ref"), case, nsteps)
+    out_gpu = _run(exe, str(tmp_path / "gpu"), case, nsteps, env={"OCFD_HIP_USER_BC": mode})
+    want = "device wall hook" if mode == "device" else "host routine ocfd_bc_User"
+    assert want in out_gpu
+    _compare_files(tmp_path, "ref", "gpu", nsteps)
+    assert _sums(out_ref) == _sums(out_gpu)
 
 
 def test_fortran_binding_config_layout(tmp_path):
