@@ -158,9 +158,9 @@ int ocfd_get_time(ocfd_handle *h, int32_t *istep, double *tt);
 /* exchange_boundary_xyz(d,u,v,w,T) (OCFD_NS_Solver.f90:35-39,
  * OCFD_mpi.f90:183-233): on one block the periodic halo copy. */
 int ocfd_exchange_primitives(ocfd_handle *h);
-/* comput_Rhybrid (OCFD_SetHybrid.f90:222-357).  counts (may be NULL) gets the
+/* comput_Rhybrid (OCFD_SetHybrid.f90:17-152).  counts (may be NULL) gets the
  * number of faces using scheme 1/2/3 and the total, as printed at
- * OCFD_SetHybrid.f90:320-342. */
+ * OCFD_SetHybrid.f90:115-135. */
 int ocfd_comput_rhybrid(ocfd_handle *h, int64_t counts[4]);
 /* du_inviscous (OCFD_inviscous.f90:53-203) */
 int ocfd_du_inviscous(ocfd_handle *h);

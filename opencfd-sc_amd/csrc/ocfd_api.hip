@@ -215,7 +215,7 @@ __global__ void k_zero_int(int *a, int n) {
   if (t < n) a[t] = 0;
 }
 
-// forced-WENO patches of the sensor (OCFD_SetHybrid.f90:293-315), with the j
+// forced-WENO patches of the sensor (OCFD_SetHybrid.f90:87-110), with the j
 // range read from Hybrid_para(7:8) as the reference's comments intend
 __global__ void k_patch(const Geom g, double *r, int ib, int ie, int jb, int je, int kb, int ke) {
   const int64_t ni = (int64_t)(ie - ib + 1) * (je - jb + 1) * (ke - kb + 1);

@@ -357,7 +357,7 @@ __device__ inline T ud7lm(const double *c, const T *x) {
   return s;
 }
 
-// set_hybrid_scheme (OCFD_SetHybrid.f90:207-220)
+// set_hybrid_scheme (OCFD_SetHybrid.f90:2-15)
 __device__ inline int hybrid_index(double r0, double t2, double t3) {
   return r0 <= t2 ? 1 : (r0 <= t3 ? 2 : 3);
 }

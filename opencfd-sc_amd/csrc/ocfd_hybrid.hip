@@ -1,6 +1,6 @@
 // ocfd_hybrid.hip -- the nonlinear faces of the hybrid scheme, gathered.
 //
-// The hybrid scheme (OCFD_Schemes.f90:319-358, OCFD_SetHybrid.f90:207-220)
+// The hybrid scheme (OCFD_Schemes.f90:319-358, OCFD_SetHybrid.f90:2-15)
 // reconstructs most faces with the linear UD7L blend and a few with WENO7 or
 // WENO5, chosen per face from the sensor Rhybrid that comput_Rhybrid forms
 // once per time step.  Computed inside the sweep tiles, the few WENO faces

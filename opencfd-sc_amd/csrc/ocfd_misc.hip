@@ -1,8 +1,8 @@
 // ocfd_misc.hip -- element-wise and small kernels of the step:
 //   periodic halo fill        exchange_boundary_x/y/z on one block (OCFD_mpi.f90:198-233)
-//   shock sensor              comput_Rhybrid (OCFD_SetHybrid.f90:222-357)
+//   shock sensor              comput_Rhybrid (OCFD_SetHybrid.f90:17-152)
 //   RK3 / primitives          OCFD_time_adv_RK3, comput_duvwT (OCFD_Time_Adv.f90:4-63)
-//   boundary conditions       OCFD_bc + Flow_Ghost_Extent_2nd (OCFD_BC.f90:2-204,
+//   boundary conditions       OCFD_bc + Flow_Ghost_Extent_2nd (OCFD_BC.f90:2-99,
 //                             OCFD_GhostCell.f90:164-286)
 //   filters                   filter_[xyz]3d[_shock] (OCFD_filtering.f90:73-363)
 //   diagnostics               show_flow_msg sums, ana_box sums (two-level,
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_halo_batch(const BoxSet S) {
 }
 
 // ---------------------------------------------------------------------------
-// shock sensor 1 (OCFD_SetHybrid.f90:240-257): R = sum over axes of
+// shock sensor 1 (OCFD_SetHybrid.f90:32-52): R = sum over axes of
 // |p(-1) - 2p + p(+1)| / |p(-1) + 2p + p(+1)|, p = p00 d T
 // ---------------------------------------------------------------------------
 // E: element type of the primitive fields
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(NT) void k_sensor1(const Block B, const double p00)
   B.rhyb[h] = r;
 }
 
-// sensor 2 (OCFD_SetHybrid.f90:259-290), pass 1: |grad p| and block sums
+// sensor 2 (OCFD_SetHybrid.f90:54-85), pass 1: |grad p| and block sums
 __global__ __launch_bounds__(NT) void k_sensor2a(const Block B, const double p00, double *partial) {
   const Geom &g = B.g;
   __shared__ double red[NT];
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(NT) void k_sensor2b(const Block B, const double *su
   B.rhyb[h] = B.rhyb[h] * dp_av1;
 }
 
-// scheme shares over x-faces (OCFD_SetHybrid.f90:320-336)
+// scheme shares over x-faces (OCFD_SetHybrid.f90:115-131)
 __global__ __launch_bounds__(NT) void k_hyb_counts(const Block B, unsigned long long *counts) {
   const Geom &g = B.g;
   const int64_t nf = (int64_t)(g.nx + 1) * g.ny * g.nz;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(NT) void k_bl_sides(const Block B, const BLArgs a) 
     for (int q = 0; q < 5; q++) F[q][g.hi(i, j, g.nz)] = F[q][g.hi(i, j, g.nz - 1)];
 }
 
-// f recomputed from the primitives on the six block faces (OCFD_BC.f90:132-200)
+// f recomputed from the primitives on the six block faces (OCFD_BC.f90:26-95)
 __global__ __launch_bounds__(NT) void k_bc_planes(const Block B) {
   if (halted(B)) return;  // a whole-step run stopped at too many T <= 0 cells
   const Geom &g = B.g;

@@ -1,6 +1,6 @@
 // ocfd_visc.h -- viscous right-hand side (du_viscous, OCFD_viscous.f90:29-149)
 // and the per-cell stage update (OCFD_time_adv_RK3, comput_duvwT and the
-// boundary-plane refresh of OCFD_bc, OCFD_Time_Adv.f90:4-63, OCFD_BC.f90:132-200).
+// boundary-plane refresh of OCFD_bc, OCFD_Time_Adv.f90:4-63, OCFD_BC.f90:26-95).
 //
 // k_visc_flux   12 central derivatives of u,v,w,T (CD6/CD8 with one-sided
 //               block-edge closures), Sutherland mu, stresses, heat flux and
@@ -295,7 +295,7 @@ __device__ inline void prims_of(double cv, const double *fq, double &d, double &
   w = fq[3] / fq[0];
   T = (fq[4] - (fq[1] * u + fq[2] * v + fq[3] * w) * 0.5) / (fq[0] * cv);
 }
-// f from primitives on the boundary planes (OCFD_BC.f90:135-139)
+// f from primitives on the boundary planes (OCFD_BC.f90:27-37)
 __device__ inline void refresh_of(double cv, double d, double u, double v, double w, double T, double *fq) {
   fq[0] = d;
   fq[1] = d * u;

@@ -6,7 +6,7 @@ primitive variables, and can write the three input files the reference solver
 reads (opencfd2.in, opencfd.dat, OCFD-grid.dat).
 
 Nondimensionalisation follows the reference: free-stream density, velocity and
-temperature are 1, p = d*T/(gamma*Ma^2) (OCFD_SetHybrid.f90:237-243), the sound
+temperature are 1, p = d*T/(gamma*Ma^2) (OCFD_SetHybrid.f90:32-41), the sound
 speed is sqrt(T)/Ma (OCFD_inviscous.f90:67) and Cv = 1/(gamma(gamma-1)Ma^2)
 (OCFD_readpara.f90:269).
 

@@ -508,7 +508,7 @@ static double ud7l_m(const double *c, const double *v) {
            c[6] * v[-2] + c[7] * v[-3];
 }
 
-/* set_hybrid_scheme (OCFD_SetHybrid.f90:207-220) */
+/* set_hybrid_scheme (OCFD_SetHybrid.f90:2-15) */
 static int set_hybrid_scheme(const oracle_block *b, double r0) {
     if (r0 <= b->c.hybrid_para[1]) return 1;
     if (r0 <= b->c.hybrid_para[2]) return 2;
@@ -1052,7 +1052,7 @@ static void bc_blunt2d(oracle_block *b) {
     }
 }
 
-/* The case hook ocfd_bc_User (OCFD_BC.f90:123-124), as supplied to the
+/* The case hook ocfd_bc_User (OCFD_BC.f90:18-19), as supplied to the
  * compiled reference by oracle/ref_user_hooks.f90: isothermal (BC_para(1) > 0)
  * or adiabatic no-slip wall at j=1 and free stream at j=ny. */
 static void bc_user(oracle_block *b) {
@@ -1141,7 +1141,7 @@ int oracle_exchange_primitives(oracle_block *b) {
 }
 
 /* ---------------------------------------------------------------------- */
-/* comput_Rhybrid (OCFD_SetHybrid.f90:222-357)                              */
+/* comput_Rhybrid (OCFD_SetHybrid.f90:17-152)                              */
 /* ---------------------------------------------------------------------- */
 int oracle_comput_rhybrid(oracle_block *b, int64_t counts[4]) {
     double p00 = 1.0 / (b->c.gamma * b->c.ma * b->c.ma);
@@ -1176,7 +1176,7 @@ int oracle_comput_rhybrid(oracle_block *b, int64_t counts[4]) {
         for (int k = 1; k <= b->nz; k++) for (int j = 1; j <= b->ny; j++) for (int i = 1; i <= b->nx; i++)
             b->rhyb[HI(b, i, j, k)] = b->rhyb[HI(b, i, j, k)] * dp_av1;
     }
-    /* forced-WENO patches (OCFD_SetHybrid.f90:293-315), with the j range read
+    /* forced-WENO patches (OCFD_SetHybrid.f90:87-110), with the j range read
      * from Hybrid_para(7:8) as the comments there intend */
     int mpatch = (int)lround(b->c.hybrid_para[3]);
 #define hp b->c.hybrid_para

@@ -1,7 +1,7 @@
 ! User hooks linked into the probe driver in place of the reference's empty
 ! OCFD_User_Def.f90 (test infrastructure only).  ocfd_bc_User is the reference's
 ! designated extension point for a case-specific boundary condition
-! (OCFD_BC.f90:123-124).  This one implements the isothermal-wall / far-field
+! (OCFD_BC.f90:18-19).  This one implements the isothermal-wall / far-field
 ! pair used by the curvilinear O-grid case:
 !   j=1  : no-slip wall, T = BC_para(1), density from a second-order
 !          extrapolation of the pressure p ~ d*T along j
