@@ -59,3 +59,23 @@ def test_create_without_gpu_fails_cleanly():
     rc = lib.ocfd_create(ctypes.byref(cfg), 0, ctypes.byref(h))
     assert rc < 0
     assert lib.ocfd_last_error()
+
+
+def test_create_refuses_hybrid_face_lists_beyond_int32():
+    """The hybrid scheme's face lists hold 2 * face id + class in an int32:
+    ocfd_create refuses a block whose faces per axis would overflow that,
+    before any device work (so this runs with or without a GPU)."""
+    lib = native.load_library()
+    from ocfd.config import CaseConfig
+    for n, refused in ((1100, True), (8, False)):
+        cfg = CaseConfig(nx=n, ny=n, nz=n, iperiodic=(1, 1, 1), scheme_invis="HYBRID").to_c()
+        h = ctypes.c_void_p()
+        rc = lib.ocfd_create(ctypes.byref(cfg), 0, ctypes.byref(h))
+        msg = lib.ocfd_last_error().decode()
+        if refused:
+            assert rc < 0 and "2^31" in msg, msg
+        else:
+            # small blocks pass the guard (without a GPU they fail later)
+            assert rc == 0 or "2^31" not in msg
+            if rc == 0:
+                lib.ocfd_destroy(h)
