@@ -205,6 +205,13 @@ int ocfd_set_bc_boundarylayer(ocfd_handle *h, double aoa, const double *inlet, c
 int ocfd_set_bc_time_factor(ocfd_handle *h, double ht);
 /* filtering(f) (OCFD_filtering.f90:4-70) at the handle's current Istep. */
 int ocfd_filtering(ocfd_handle *h);
+/* dt = cfl / max over the cells of sum over the axes of
+ * (|u . grad xi| + c |grad xi|) / h_xi, c = sqrt(T)/Ma, from the device state
+ * and metrics (a wavefront / workgroup max-reduction).  No counterpart in the
+ * reference, which advances with the fixed Para%dt (OCFD_Time_Adv.f90:20, 30);
+ * a caller computes dt once with it and puts it in ocfd_config.dt.  The
+ * handle's dt is not changed. */
+int ocfd_cfl_dt(ocfd_handle *h, double cfl, double *dt);
 /* Advance Istep by one and tt by dt (OCFD_NS_Solver.f90:64-65). */
 int ocfd_advance_clock(ocfd_handle *h);
 

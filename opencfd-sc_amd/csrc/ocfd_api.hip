@@ -901,6 +901,21 @@ int ocfd_set_bc_time_factor(ocfd_handle *h, double ht) {
   return 0;
 }
 
+int ocfd_cfl_dt(ocfd_handle *h, double cfl, double *dt) {
+  if (!dt || !(cfl > 0.0)) return fail("ocfd_cfl_dt: cfl must be > 0 and dt non-NULL");
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(h->wout);
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, h->st, h->wout, (int64_t)1, 0.0);
+  if (launch(h, "cfl_rate", [&] { return launch_cfl_rate(h->B, w, h->st); })) return -1;
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, w, sizeof bits, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double rate;
+  std::memcpy(&rate, &bits, sizeof rate);
+  if (!(rate > 0.0) || !std::isfinite(rate)) return fail("ocfd_cfl_dt: no finite positive rate in the block");
+  *dt = cfl / rate;
+  return 0;
+}
+
 int ocfd_advance_clock(ocfd_handle *h) {
   h->istep += 1;
   h->tt = h->tt + h->cfg.dt;

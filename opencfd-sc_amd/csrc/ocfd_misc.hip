@@ -195,6 +195,40 @@ __global__ __launch_bounds__(NT) void k_hyb_counts(const Block B, unsigned long 
 }
 
 // ---------------------------------------------------------------------------
+// CFL time step: the largest over the cells of
+//   sum over axes a of (|u . A_a| + c |A_a|) / h_a,   c = sqrt(T) / Ma,
+// with A_a = (dxi_a/dx, dxi_a/dy, dxi_a/dz) the metric row of axis a.  The
+// reference advances with a fixed Para%dt (OCFD_Time_Adv.f90:20, 30); this is
+// the dt = CFL / rate a case generator would fix.  Positive doubles order as
+// their bit patterns, so the block maxima combine with an integer atomicMax.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_cfl_rate(const Block B, unsigned long long *out) {
+  const Geom &g = B.g;
+  double m = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < g.ni; t += (int64_t)gridDim.x * NT) {
+    int i, j, k;
+    cell_of(g, t, i, j, k);
+    const int64_t h = g.hi(i, j, k);
+    const double u = B.u[h], v = B.v[h], w = B.w[h];
+    const double c = sqrt(double(B.T[h])) / B.P.ma;
+    double r = 0.0;
+    for (int a = 0; a < 3; a++) {
+      const double ax = B.M.get(3 * a, h), ay = B.M.get(3 * a + 1, h), az = B.M.get(3 * a + 2, h);
+      r += (fabs(u * ax + v * ay + w * az) + c * sqrt(ax * ax + ay * ay + az * az)) * B.P.hinv[a];
+    }
+    m = fmax(m, r);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  __shared__ double red[NT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < NT / 64; q++) m = fmax(m, red[q]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // RK3 stage (OCFD_Time_Adv.f90:4-36) / fn = f / init3d f / comput_duvwT
 // ---------------------------------------------------------------------------
 // du is held in DU (double; float in the mixed-precision form); the stage
@@ -1022,6 +1056,12 @@ int64_t halo_copy_tiles(const Geom &g) { return halo_copy_tiles_of(g); }
 
 hipError_t launch_negt_repair_batch(const BoxSet &S, int nbox, int *neg, int ntiles, hipStream_t st) {
   hipLaunchKernelGGL(k_negt_repair_batch, dim3((unsigned)(nbox + ntiles)), dim3(NT), 0, st, S, nbox, neg);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl_rate(const Block &B, unsigned long long *out, hipStream_t st) {
+  const int64_t nblk = (B.g.ni + NT - 1) / NT;
+  hipLaunchKernelGGL(k_cfl_rate, dim3((unsigned)(nblk < 2048 ? nblk : 2048)), dim3(NT), 0, st, B, out);
   return hipGetLastError();
 }
 

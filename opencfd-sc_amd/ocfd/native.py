@@ -296,6 +296,12 @@ class Block(BlockBase):
         return self._call("synchronize")
 
     # -- sweep tiles ------------------------------------------------------------
+    def cfl_dt(self, cfl):
+        """ocfd_cfl_dt: dt = cfl / (largest CFL rate over the cells)."""
+        dt = ctypes.c_double()
+        self._call("cfl_dt", ctypes.c_double(cfl), ctypes.byref(dt))
+        return dt.value
+
     def set_sweep_tiles(self, shapes):
         a = (ctypes.c_int32 * 3)(*[int(x) for x in shapes])
         return self._call("set_sweep_tiles", a)
