@@ -236,6 +236,16 @@ int ocfd_autotune(ocfd_handle *h, int32_t reps, int32_t shape_out[3]);
 /* show_flow_msg sums (OCFD_Time_Adv.f90:65-90): out[0..2] = E, K, S
  * (averaged over the global cell count). */
 int ocfd_show_flow_msg(ocfd_handle *h, double out[3]);
+/* ocfd_show_flow_msg that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial sums live in the library's file-scope device array of 16 MiB
+ * (2^21 doubles, held by every GPU the library runs on until the process
+ * ends), so blocks of at most about 178.9 million cells (3 doubles per 256
+ * cells) are accepted; larger ones fail with a message.  Results equal
+ * ocfd_show_flow_msg's bit for bit.  All calls on one GPU share the array:
+ * calls that overlap on one GPU race. */
+int ocfd_show_flow_msg_noalloc(ocfd_handle *h, double out[3]);
 /* ana_box statistics (OCFD_ANA.f90:178-311): out[0..19] in the column order
  * of statistics.dat: tt, Re_lamda, Re_lamdax, Amt, vv_rms, u_rms, v_rms,
  * w_rms, u_skewness, u_flatness, ux_skewness, ux_flatness, E_kinetic,

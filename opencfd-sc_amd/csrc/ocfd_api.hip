@@ -12,6 +12,7 @@
 #include <functional>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,11 @@
 #include "ocfd_kernels.h"
 
 using namespace ocfd;
+
+// Scratch of the entry points that allocate nothing (see noalloc_scratch)
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = 2 * 1024 * 1024;  // 16 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
 
 namespace {
 thread_local std::string g_err = "";
@@ -74,6 +80,41 @@ namespace {
     hipError_t e_ = (expr);                                                                            \
     if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_));            \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// Scratch of the entry points that allocate nothing (ocfd_show_flow_msg_noalloc):
+// a file-scope device array that every GPU the library runs on holds, at its
+// own address, until the process ends.  Host code must not pass `scratch`
+// itself to a kernel (in host code it names a host-side stand-in); it uses
+// the device address hipGetSymbolAddress returns with that device current,
+// looked up once per device and kept in g_scratch_ptr.  Calls on one GPU
+// share the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// 3 partial sums per workgroup of the show_flow_msg reduction, then the 3 totals
+inline int64_t noalloc_flow_msg_doubles(const Geom &g) { return 3 * flow_msg_partials(g) + 3; }
+
+// the device address of `scratch` on `dev`, looked up with dev current on
+// the first call for that device
+int noalloc_scratch(int dev, double **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail("no-allocation scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
 
 hipEvent_t ev_get(ocfd_handle *h) {
   if (!h->pool.empty()) {
@@ -1112,6 +1153,28 @@ int ocfd_show_flow_msg(ocfd_handle *h, double out[3]) {
   HIPCHK(hipStreamSynchronize(h->st));
   const double n = (double)(h->cfg.nx * h->cfg.ny * h->cfg.nz);
   for (int q = 0; q < 3; q++) out[q] = s[q] / n;
+  return 0;
+}
+
+// show_flow_msg with the partial sums in the library's file-scope device array
+// (no allocation).  k_flow_msg writes every partial and k_sum_partials every
+// total, so nothing in the array has to start at zero.
+int ocfd_show_flow_msg_noalloc(ocfd_handle *h, double out[3]) {
+  if (!h || !out) return fail("ocfd_show_flow_msg_noalloc: NULL argument");
+  const int64_t need = noalloc_flow_msg_doubles(h->B.g);
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return fail("ocfd_show_flow_msg_noalloc: a block of " + std::to_string(h->B.g.ni) + " cells needs " +
+                std::to_string(need) + " doubles of scratch, more than the " +
+                std::to_string(NOALLOC_SCRATCH_DOUBLES) + " it has (at most about 178.9 million cells)");
+  double *s = nullptr;
+  if (noalloc_scratch(h->device, &s)) return -1;
+  double *tot = s + 3 * flow_msg_partials(h->B.g);
+  HIPCHK(launch_reduce_flow_msg(h->B, s, tot, h->st));
+  double r[3];
+  HIPCHK(hipMemcpyAsync(r, tot, sizeof r, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const double n = (double)(h->cfg.nx * h->cfg.ny * h->cfg.nz);
+  for (int q = 0; q < 3; q++) out[q] = r[q] / n;
   return 0;
 }
 

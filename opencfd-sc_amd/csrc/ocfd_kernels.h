@@ -240,6 +240,8 @@ hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st);
 // T <= 0 repair of the cells listed in neg (T2 from T2 / T neighbours) and
 // the copy of T's halo into T2, in one launch
 hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st);
+// workgroups of the show_flow_msg reduction (3 partial sums each)
+int64_t flow_msg_partials(const Geom &g);
 // largest CFL rate over the cells, as the bit pattern of a positive double,
 // combined into *out with atomicMax (*out must hold 0 before)
 hipError_t launch_cfl_rate(const Block &B, unsigned long long *out, hipStream_t st);

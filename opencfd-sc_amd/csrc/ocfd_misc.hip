@@ -1119,6 +1119,8 @@ hipError_t launch_filter(const Block &B, int axis, int scheme, double s0, double
   return hipGetLastError();
 }
 
+int64_t flow_msg_partials(const Geom &g) { return nb(g.ni); }
+
 hipError_t launch_reduce_flow_msg(const Block &B, double *partial, double *out, hipStream_t st) {
   const unsigned nblk = nb(B.g.ni);
   hipLaunchKernelGGL(k_flow_msg, dim3(nblk), dim3(NT), 0, st, B, partial);

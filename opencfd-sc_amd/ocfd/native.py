@@ -295,6 +295,13 @@ class Block(BlockBase):
     def synchronize(self):
         return self._call("synchronize")
 
+    def show_flow_msg_noalloc(self):
+        """ocfd_show_flow_msg_noalloc: E, K, S with the partial sums in the
+        library's file-scope device scratch (no allocation)."""
+        out = (ctypes.c_double * 3)()
+        self._call("show_flow_msg_noalloc", out)
+        return np.array(out[:])
+
     # -- sweep tiles ------------------------------------------------------------
     def cfl_dt(self, cfl):
         """ocfd_cfl_dt: dt = cfl / (largest CFL rate over the cells)."""
