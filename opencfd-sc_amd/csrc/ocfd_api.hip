@@ -1081,7 +1081,7 @@ int ocfd_step(ocfd_handle *h, int32_t nsteps) {
 // ---- sweep tiles ---------------------------------------------------------------
 int ocfd_set_sweep_tiles(ocfd_handle *h, const int32_t shape[3]) {
   for (int a = 0; a < 3; a++)
-    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0, 1 or 2");
+    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0 .. 4");
   for (int a = 0; a < 3; a++) h->tile[a] = shape[a];
   invalidate_graph(h);
   return 0;
@@ -1114,10 +1114,12 @@ int ocfd_autotune(ocfd_handle *h, int32_t reps, int32_t shape_out[3]) {
   rc = 0;
   for (int a = 0; a < 3 && !rc; a++) {
     // the alternative shapes exist for periodic axes with uniform metrics
-    const int nshape = (c.iperiodic[a] == 1 && B.M.uniform) ? NSWEEP_SHAPES : 1;
+    // (the marches for the y and z sweeps of some schemes)
+    const bool fast = c.iperiodic[a] == 1 && B.M.uniform;
     float best = 0.f;
     int bests = 0;
-    for (int sh = 0; sh < nshape && !rc; sh++) {
+    for (int sh = 0; sh < NSWEEP_SHAPES && !rc; sh++) {
+      if (!sweep_shape_supported(a, sh, fast, c.scheme_invis, ch)) continue;
       hipError_t e = launch_inviscid(B, a, modes[a], SweepTile{sh}, h->st);  // warm-up
       e = e != hipSuccess ? e : hipEventRecord(e0, h->st);
       for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_inviscid(B, a, modes[a], SweepTile{sh}, h->st);
@@ -1326,7 +1328,8 @@ void batch_free(ocfd_batch *b) {
 int batch_sweep(ocfd_batch *b, const ocfd_batch::SweepGroup &g, int a, int shape, int table = 0) {
   static const int modes[3] = {INV_ASSIGN, INV_ACCUM, INV_FINAL};
   const ocfd_batch::Launch &l = g.l[a][shape];
-  const hipError_t e = launch_inviscid_batch(boxset(b, l, table), b->prec, a, g.sch, g.ch, modes[a], shape,
+  const int eff = sweep_shape_effective(a, shape, true, g.sch, g.ch);
+  const hipError_t e = launch_inviscid_batch(boxset(b, l, table), b->prec, a, g.sch, g.ch, modes[a], eff,
                                              (int)l.ntiles, b->st);
   return e == hipSuccess ? 0 : fail(std::string("batched sweep: ") + hipGetErrorString(e));
 }
@@ -1465,7 +1468,9 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   for (auto &g : b->sweep)
     for (int a = 0; a < 3; a++)
       for (int sh = 0; sh < NSWEEP_SHAPES; sh++)
-        g.l[a][sh] = add_launch(b, g.ids, [a, sh](const Geom &gg) { return sweep_tiles(gg, a, sh); });
+        g.l[a][sh] = add_launch(b, g.ids, [a, sh, &g](const Geom &gg) {
+          return sweep_tiles(gg, a, sweep_shape_effective(a, sh, true, g.sch, g.ch));
+        });
   for (auto &v : b->vis)
     for (int q = 0; q < NBATCH_KSEG; q++)
       v.l[q] = add_launch(b, v.ids, [q](const Geom &g) { return march_tiles(g, BATCH_KSEGS[q]); });
@@ -1568,7 +1573,7 @@ int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
 int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape[3]) {
   if (group < 0 || group >= (int)b->sweep.size()) return fail("sweep group out of range");
   for (int a = 0; a < 3; a++)
-    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0, 1 or 2");
+    if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0 .. 4");
   for (int a = 0; a < 3; a++) b->sweep[group].shape[a] = shape[a];
   batch_invalidate(b);
   return 0;
@@ -1597,6 +1602,7 @@ int ocfd_batch_autotune(ocfd_batch *b, int32_t reps) {
       float best = 0.f;
       int bests = 0;
       for (int sh = 0; sh < NSWEEP_SHAPES && !rc; sh++) {
+        if (!sweep_shape_supported(a, sh, true, g.sch, g.ch)) continue;
         rc = batch_sweep(b, g, a, sh);  // warm-up
         if (!rc && hipEventRecord(e0, b->st) != hipSuccess) rc = fail("hipEventRecord failed");
         for (int r = 0; r < reps && !rc; r++) rc = batch_sweep(b, g, a, sh);

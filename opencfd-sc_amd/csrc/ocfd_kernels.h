@@ -165,20 +165,50 @@ enum InvMode : int { INV_ASSIGN = 0, INV_ACCUM = 1, INV_FINAL = 2 };
 
 // Launch shape of one sweep: 0 is the built-in tile of each axis (x 64x4,
 // y and z 32x8 points x lines); 1 (x 32x8, y/z 16x16) and 2 (x 128x2, y/z
-// 64x4) exist for periodic axes with uniform metrics and are chosen by the
-// autotuner (ocfd_autotune).
-constexpr int NSWEEP_SHAPES = 3;
+// 64x4) exist for periodic axes with uniform metrics; 3 and 4 are the
+// marches (one lane per line segment, ocfd_march.h: segments of 32 / 64
+// cells for y and z, runs of 8 / 16 cells for x) of periodic axes with
+// uniform metrics, for the non-characteristic schemes and the characteristic
+// hybrid scheme.  The autotuner (ocfd_autotune) chooses
+// among the shapes a sweep supports; a shape a sweep does not support runs
+// as shape 0.
+constexpr int NSWEEP_SHAPES = 5;
 struct SweepTile {
   int shape;
 };
-// points along the axis of a tile of `shape` (256 threads: lines = 256 / TL)
+// cells per segment of march shape `shape` (3, 4) along `axis`
+inline int march_segment(int axis, int shape) {
+  return axis == 0 ? (shape == 4 ? 16 : 8) : (shape == 4 ? 64 : 32);
+}
+// whether `shape` runs on a sweep of `axis` (fast: periodic axis, uniform
+// metrics; sch, ch: scheme and characteristic flag)
+inline bool sweep_shape_supported(int axis, int shape, bool fast, int sch, bool ch) {
+  if (shape == 0) return true;
+  if (!fast) return false;
+  if (shape <= 2) return true;
+  return !ch || sch == SCH_HYBRID;
+}
+// the shape that actually runs
+inline int sweep_shape_effective(int axis, int shape, bool fast, int sch, bool ch) {
+  return sweep_shape_supported(axis, shape, fast, sch, ch) ? shape : 0;
+}
+// points along the axis of a tile of `shape` 0-2 (256 threads: lines = 256 / TL)
 inline int sweep_shape_tl(int axis, int shape) {
   return axis == 0 ? (shape == 1 ? 32 : (shape == 2 ? 128 : 64)) : (shape == 1 ? 16 : (shape == 2 ? 64 : 32));
 }
-// workgroups of one sweep of a block
+// workgroups of one sweep of a block in an effective shape
 inline int64_t sweep_tiles(const Geom &g, int axis, int shape) {
-  const int tl = sweep_shape_tl(axis, shape), tw = 256 / tl;
   const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  if (shape >= 3) {
+    const int sl = march_segment(axis, shape);
+    const int nseg = (n + sl - 1) / sl;
+    // x: one lane per run, 256 lanes per workgroup
+    if (axis == 0) return ((int64_t)nseg * g.ny * g.nz + 255) / 256;
+    // y, z: 4 segments of 64 lines per workgroup
+    const int nsg = (nseg + 3) / 4;
+    return (int64_t)((g.nx + 63) / 64) * nsg * (axis == 2 ? g.ny : g.nz);
+  }
+  const int tl = sweep_shape_tl(axis, shape), tw = 256 / tl;
   const int nw = axis == 0 ? g.ny : g.nx;
   const int nc = axis == 2 ? g.ny : g.nz;
   return (int64_t)((n + tl - 1) / tl) * ((nw + tw - 1) / tw) * nc;

@@ -19,6 +19,7 @@
 // that includes this header instantiates one R.
 #pragma once
 #include "ocfd_kernels.h"
+#include "ocfd_march.h"
 
 namespace ocfd {
 
@@ -526,11 +527,39 @@ hipError_t dispatch_scheme(const Block &B, int mode, hipStream_t st) {
   }
 }
 
+// the march (shapes 3, 4) of one scheme
+template <int AXIS, int SCH, bool CHAR, typename R>
+hipError_t launch_march(const Block &B, int mode, int shape, hipStream_t st) {
+  if constexpr (!march_supported<SCH, CHAR>()) {
+    return hipErrorInvalidValue;
+  } else {
+    const int sl = march_segment(AXIS, shape);
+    hipLaunchKernelGGL((k_inviscid_march<AXIS, SCH, CHAR, R>), dim3((unsigned)sweep_tiles(B.g, AXIS, shape)),
+                       dim3(64 * MARCH_WAVES), 0, st, B, mode, sl);
+    return hipGetLastError();
+  }
+}
+
+template <int AXIS, bool CHAR, typename R>
+hipError_t dispatch_march(const Block &B, int mode, int shape, hipStream_t st) {
+  switch (B.S.scheme_invis) {
+    case SCH_WENO5: return launch_march<AXIS, SCH_WENO5, CHAR, R>(B, mode, shape, st);
+    case SCH_WENO7: return launch_march<AXIS, SCH_WENO7, CHAR, R>(B, mode, shape, st);
+    case SCH_OMP6: return launch_march<AXIS, SCH_OMP6, CHAR, R>(B, mode, shape, st);
+    case SCH_UD7L: return launch_march<AXIS, SCH_UD7L, CHAR, R>(B, mode, shape, st);
+    case SCH_HYBRID: return launch_march<AXIS, SCH_HYBRID, CHAR, R>(B, mode, shape, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <int AXIS, typename R>
 hipError_t dispatch_axis(const Block &B, int mode, int shape, hipStream_t st) {
   const bool ch = B.S.is_char != 0;
   const bool fast = B.S.iperiodic[AXIS] == 1 && B.M.uniform;
+  shape = sweep_shape_effective(AXIS, shape, fast, B.S.scheme_invis, ch);
   if (fast) {
+    if (shape >= 3)
+      return ch ? dispatch_march<AXIS, true, R>(B, mode, shape, st) : dispatch_march<AXIS, false, R>(B, mode, shape, st);
     if (shape == 1)
       return ch ? dispatch_scheme<AXIS, true, true, R, 1>(B, mode, st)
                 : dispatch_scheme<AXIS, false, true, R, 1>(B, mode, st);
@@ -573,8 +602,30 @@ hipError_t batch_axis(const BoxSet &S, int sch, bool ch, int mode, int ntiles, h
   return hipGetLastError();
 }
 
+// the march of a batch group (no hybrid in batches)
+template <int AXIS, typename R>
+hipError_t batch_march(const BoxSet &S, int sch, int mode, int shape, int ntiles, hipStream_t st) {
+  {
+    const int sl = march_segment(AXIS, shape);
+#define OCFD_BATCH_MARCH(SCH_)                                                                              \
+  hipLaunchKernelGGL((k_inviscid_march_batch<AXIS, SCH_, false, R>), dim3((unsigned)ntiles), dim3(64 * MARCH_WAVES), \
+                     0, st, S, mode, sl)
+    switch (sch) {
+      case SCH_WENO5: OCFD_BATCH_MARCH(SCH_WENO5); break;
+      case SCH_WENO7: OCFD_BATCH_MARCH(SCH_WENO7); break;
+      case SCH_OMP6: OCFD_BATCH_MARCH(SCH_OMP6); break;
+      case SCH_UD7L: OCFD_BATCH_MARCH(SCH_UD7L); break;
+      default: return hipErrorInvalidValue;
+    }
+#undef OCFD_BATCH_MARCH
+    return hipGetLastError();
+  }
+}
+
+// shape: the effective shape of the group (sweep_shape_effective)
 template <int AXIS, typename R>
 hipError_t batch_shape(const BoxSet &S, int sch, bool ch, int mode, int shape, int ntiles, hipStream_t st) {
+  if (shape >= 3) return ch ? hipErrorInvalidValue : batch_march<AXIS, R>(S, sch, mode, shape, ntiles, st);
   if (shape == 1) return batch_axis<AXIS, R, 1>(S, sch, ch, mode, ntiles, st);
   if (shape == 2) return batch_axis<AXIS, R, 2>(S, sch, ch, mode, ntiles, st);
   return batch_axis<AXIS, R, 0>(S, sch, ch, mode, ntiles, st);

@@ -77,6 +77,22 @@ def test_batch_equals_single_boxes(precision):
         assert batch.box(0).get_time()[0] == 5
 
 
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_MIXED], ids=["fp64", "mixed"])
+def test_batch_march_shapes(precision):
+    """Every group on the marches (shapes 3, 4; the characteristic groups
+    run them as shape 0): bitwise equal to the boxes stepped alone."""
+    cs = make_cases(precision)
+    ref3 = single_runs(cs, 3)
+    with native.Batch([c.cfg for c in cs]) as batch:
+        for i, c in enumerate(cs):
+            native.setup_block(batch.box(i), c, use_uniform=True)
+        ngroups = batch.info()[0]
+        for gi in range(ngroups):
+            batch.set_sweep_tiles(gi, [3 + gi % 2, 3 + gi % 2, 4 - gi % 2])
+        assert batch.step(3) == 0
+        check_equal(batch, ref3, "3 batched steps on the marches")
+
+
 def test_batch_rejects_unsupported_boxes():
     c = cases.random_case(shape=(16, 16, 16), seed=87, scheme="HYBRID", split=2, char=1)
     c.cfg = c.cfg.replace(metric_mode=1)

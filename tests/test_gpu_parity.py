@@ -261,10 +261,32 @@ def test_sweep_tile_shapes_bitwise(kw):
         du0 = _rhs(ref, case)
         ref.step(2)
         f0 = ref.download_f()
-    for shapes in ([1, 1, 1], [2, 2, 2], [1, 2, 0]):
+    for shapes in ([1, 1, 1], [2, 2, 2], [1, 2, 0], [3, 3, 3], [4, 4, 4], [1, 3, 4], [3, 0, 4]):
         with gpu_block(case, uniform=True) as b:
             b.set_sweep_tiles(shapes)
             assert b.get_sweep_tiles() == shapes
+            assert_same(_rhs(b, case), du0, "du with tiles %s" % shapes)
+            b.step(2)
+            assert_same(b.download_f(), f0, "f after 2 steps with tiles %s" % shapes)
+
+
+@pytest.mark.parametrize("kw", [UNIFORM[0], UNIFORM[4], UNIFORM[5]], ids=["HYBRIDc", "HYBRIDnc", "WENO5"])
+def test_march_shapes_wide_block(kw):
+    """The marches (shapes 3, 4) on a block whose i extent needs two
+    wavefronts in y and z, the second one part idle, and whose extents end
+    in partial segments and runs: bitwise equal to the tile sweeps."""
+    kw = dict(kw)
+    kw.pop("filters", None)
+    case = cases.random_case(shape=(70, 36, 45), **kw)
+    case.uniform_metrics = cases.uniform_box_metrics(case.cfg.nx, case.cfg.ny, case.cfg.nz)
+    case.cfg.metric_mode = 1
+    with gpu_block(case, uniform=True) as ref:
+        du0 = _rhs(ref, case)
+        ref.step(2)
+        f0 = ref.download_f()
+    for shapes in ([3, 3, 4], [4, 4, 3]):
+        with gpu_block(case, uniform=True) as b:
+            b.set_sweep_tiles(shapes)
             assert_same(_rhs(b, case), du0, "du with tiles %s" % shapes)
             b.step(2)
             assert_same(b.download_f(), f0, "f after 2 steps with tiles %s" % shapes)
@@ -276,7 +298,7 @@ def test_autotune_keeps_results():
     case.cfg.metric_mode = 1
     with gpu_block(case, uniform=True) as ref, gpu_block(case, uniform=True) as b:
         shapes = b.autotune(reps=2)
-        assert all(s in (0, 1, 2) for s in shapes) and b.get_sweep_tiles() == shapes
+        assert all(s in range(5) for s in shapes) and b.get_sweep_tiles() == shapes
         ref.step(2)
         b.step(2)
         assert_same(b.download_f(), ref.download_f(), "f after autotune + 2 steps (tiles %s)" % shapes)
