@@ -286,38 +286,11 @@ def step_roofline(rate_mcells, cfg, variant, mix, prec):
 # ---------------------------------------------------------------------------
 # batched ensemble of independent periodic boxes
 # ---------------------------------------------------------------------------
-BATCH_SIZES = (32, 48, 64, 96, 128)
-
-
-def batch_cases(nbox: int, precision: int, seed: int = 5):
-    """nbox boxes: n^3 with n uniform over BATCH_SIZES, a Taylor-Green or a
-    seeded isotropic initial field (seed 20261202 + box), Ma uniform in
-    [0.1, 0.5], Re uniform in [400, 3200], WENO5 or OMP6 with Steger-Warming
-    splitting (component-wise), CD6, CFL 0.5, uniform-box metrics."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(nbox):
-        n = int(rng.choice(BATCH_SIZES))
-        kind = "tgv" if rng.random() < 0.5 else "iso"
-        ma = float(rng.uniform(0.1, 0.5))
-        re = float(rng.uniform(400.0, 3200.0))
-        sch = "WENO5" if rng.random() < 0.5 else "OMP6"
-        if kind == "tgv":
-            c = cases.tgv(n, ma=ma, re=re, scheme=sch, split=1, metric_mode=METRIC_UNIFORM)
-        else:
-            c = cases.isotropic(n, seed=20261202 + i, mt=ma, re=re, scheme=sch, split=1, char=0,
-                                metric_mode=METRIC_UNIFORM)
-        c.cfg = c.cfg.replace(precision=precision)
-        out.append(c)
-    return out
-
-
 def run_batch(args):
     nbox = int(args.workload[5:] or 96)
     prec = PREC_MIXED if args.precision == "mixed" else PREC_FP64
     t0 = time.time()
-    cs = batch_cases(nbox, prec)
+    cs = cases.batch_ensemble(nbox, prec)
     cells = sum(c.cfg.nx * c.cfg.ny * c.cfg.nz for c in cs)
     log("%d boxes (%.1f Mcells) built in %.1f s" % (nbox, cells / 1e6, time.time() - t0))
     with native.Batch([c.cfg for c in cs]) as batch:
@@ -348,7 +321,7 @@ def run_batch(args):
         "dtype": "f64" if prec == PREC_FP64 else "f32+f64",
         "data": "synthetic (seeded initial fields generated on the host)",
         "config": {"workload": "batched ensemble of %d independent periodic boxes" % nbox,
-                   "box_sizes": {str(n): sizes.count(n) for n in BATCH_SIZES}, "cells": cells,
+                   "box_sizes": {str(n): sizes.count(n) for n in cases.BATCH_SIZES}, "cells": cells,
                    "initial": "Taylor-Green or seeded isotropic per box; Ma U[0.1,0.5], Re U[400,3200]",
                    "scheme": "WENO5 or OMP6 per box, Steger-Warming, CD6, RK3 CFL 0.5",
                    "precision": "FP64" if prec == PREC_FP64 else "mixed",
@@ -462,8 +435,16 @@ def cpu_baseline(workload: str, mode: str):
 
 # ---------------------------------------------------------------------------
 def tiles_desc(b):
-    return ["%dx%d" % t for t in [((64, 32, 128)[s], 256 // (64, 32, 128)[s]) if a == 0 else
-                                  ((32, 16, 64)[s], 256 // (32, 16, 64)[s]) for a, s in enumerate(b.tiles)]]
+    """Sweep launch shapes per axis: tiles (points x lines) or y/z marches
+    (one lane per line, segments of 32 or 64 cells)."""
+    out = []
+    for a, s in enumerate(b.tiles):
+        if s >= 3:
+            out.append("march%d" % (32 if s == 3 else 64))
+        else:
+            tl = (64, 32, 128)[s] if a == 0 else (32, 16, 64)[s]
+            out.append("%dx%d" % (tl, 256 // tl))
+    return out
 
 
 def dominant(prof):

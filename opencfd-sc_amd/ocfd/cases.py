@@ -372,18 +372,38 @@ def negative_t_case(shape=(8, 10, 14), seed=44, amp=4.0, t_factor=0.1, dt_factor
 
 
 def blunt_ogrid(nx=512, ny=256, nz=8, r0=1.0, dr0=None, ratio=1.05, zspan=0.1, ma=6.0,
-                re=1.0e5, t_inf=79.0, t_wall=None, cfl=0.5, filter_every=1, filter_s0=0.2) -> Case:
+                re=1.0e5, t_inf=79.0, t_wall=None, cfl=0.3, filter_every=1, filter_s0=0.2,
+                wake_at_rest=True, layer_cells=24.0) -> Case:
     """Mach-6 flow over a cylinder: O-grid (i around the body, periodic; j
     radial, wall at j=1 with ratio-`ratio` stretching; z periodic).
     Isothermal wall through the BC_USER_DEF hook with ghost cells at j-,
-    WENO5 characteristic + Steger-Warming, Fo9p filtering."""
+    WENO5 characteristic + Steger-Warming, Fo9p filtering.
+
+    Grid: first radial spacing dr0 (1e-3 r0), growing by `ratio` per cell up
+    to the circumferential spacing at 2 r0, then uniform (outer radius about
+    6.1 r0 at 512 x 256); z span `zspan`.  Wall temperature t_wall = 300 K /
+    T_inf.  Fo9p filter with s0 = filter_s0 every `filter_every` steps.
+
+    Initial state: free stream (u = 1) upstream of the body and fluid at rest
+    behind it, blended by 0.5 (1 - tanh(x / r0)), inside a wall layer (u and
+    T rise from 0 and t_wall over about `layer_cells` radial cells, uniform
+    pressure).  A uniform Mach-6 start (wake_at_rest=False) is not runnable
+    on the full O-grid: behind the body the flow leaves the wall faster than
+    the escape speed 2c/(gamma - 1) = 5c of an unsteady expansion, so T
+    reaches 0 in the first stage (the reference stops there too).  With
+    these defaults (layer of 24 cells, CFL 0.3) the reference runs the first
+    100 steps without a T <= 0 cell, so the run does not depend on how the
+    reference's per-rank T <= 0 count and repair see its MPI decomposition."""
     if dr0 is None:
-        dr0 = 5.0e-3 * (ratio - 1.0) / (ratio ** (ny - 1) - 1.0) * 200.0
+        dr0 = 1.0e-3 * r0
+    # radial spacing: dr0 at the wall, growing by `ratio` per cell up to the
+    # circumferential spacing at 2 r0, then uniform
+    dr_max = 2.0 * PI * (2.0 * r0) / nx
     th = 2.0 * PI * np.arange(nx) / nx
     r = np.empty(ny)
     r[0] = r0
     for j in range(1, ny):
-        r[j] = r[j - 1] + dr0 * ratio ** (j - 1)
+        r[j] = r[j - 1] + min(dr0 * ratio ** (j - 1), dr_max)
     x2 = -(r[:, None] * np.cos(th[None, :]))
     y2 = r[:, None] * np.sin(th[None, :])
     z1 = zspan * np.arange(nz) / nz
@@ -393,14 +413,23 @@ def blunt_ogrid(nx=512, ny=256, nz=8, r0=1.0, dr0=None, ratio=1.05, zspan=0.1, m
     if t_wall is None:
         t_wall = 300.0 / t_inf
     shape = (nz, ny, nx)
-    d = np.ones(shape)
-    u = np.ones(shape)
     v = np.zeros(shape)
     w = np.zeros(shape)
-    T = np.ones(shape)
-    # start from free stream with a no-slip, isothermal layer at the wall
-    u[:, 0, :] = 0.0
-    T[:, 0, :] = t_wall
+    if wake_at_rest:
+        delta = layer_cells * dr0
+        layer = np.tanh((r - r0) / delta)[:, None] * np.ones((1, nx))
+        blend = 0.5 * (1.0 - np.tanh(x2 / r0))
+        T2 = t_wall + (1.0 - t_wall) * layer
+        u = np.broadcast_to(layer * blend, shape).copy()
+        T = np.broadcast_to(T2, shape).copy()
+        d = 1.0 / T
+    else:
+        d = np.ones(shape)
+        u = np.ones(shape)
+        T = np.ones(shape)
+        # start from free stream with a no-slip, isothermal layer at the wall
+        u[:, 0, :] = 0.0
+        T[:, 0, :] = t_wall
     cfg = CaseConfig(nx=nx, ny=ny, nz=nz, iperiodic=(1, 0, 1), scheme_invis="WENO5",
                      scheme_vis="CD6", flux_splitting=1, if_scheme_character=1, ma=ma, re=re,
                      ref_amu_t0=t_inf, grid_type=GRID2D_PLANE, boundary_condition="BC_USER_DEF",
@@ -410,7 +439,8 @@ def blunt_ogrid(nx=512, ny=256, nz=8, r0=1.0, dr0=None, ratio=1.05, zspan=0.1, m
                      periodic_span=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, zspan))
     case = Case(cfg=cfg, x=X, y=Y, z=Z, spans=np.array(cfg.periodic_span), prims=[d, u, v, w, T],
                 grid_payload=(x2, y2, z1), name="blunt%dx%dx%d" % (nx, ny, nz),
-                info={"dr0": dr0, "ratio": ratio, "t_wall": t_wall, "zspan": zspan})
+                info={"dr0": dr0, "dr_max": dr_max, "r_outer": float(r[-1]), "ratio": ratio, "t_wall": t_wall,
+                      "zspan": zspan})
     cfg.dt = cfl_dt(case, cfl)
     return case
 
@@ -553,3 +583,33 @@ def _separable(X, Y, Z):
     return (np.array_equal(X, np.broadcast_to(X[:1, :1, :], X.shape))
             and np.array_equal(Y, np.broadcast_to(Y[:1, :, :1], Y.shape))
             and np.array_equal(Z, np.broadcast_to(Z[:, :1, :1], Z.shape)))
+
+
+# ---------------------------------------------------------------------------
+# batched ensemble of independent periodic boxes
+# ---------------------------------------------------------------------------
+BATCH_SIZES = (32, 48, 64, 96, 128)
+
+
+def batch_ensemble(nbox: int = 96, precision: int = 0, seed: int = 5):
+    """nbox boxes: n^3 with n uniform over BATCH_SIZES, a Taylor-Green or a
+    seeded isotropic initial field (seed 20261202 + box), Ma uniform in
+    [0.1, 0.5], Re uniform in [400, 3200], WENO5 or OMP6 with Steger-Warming
+    splitting (component-wise), CD6, CFL 0.5, uniform-box metrics."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nbox):
+        n = int(rng.choice(BATCH_SIZES))
+        kind = "tgv" if rng.random() < 0.5 else "iso"
+        ma = float(rng.uniform(0.1, 0.5))
+        re = float(rng.uniform(400.0, 3200.0))
+        sch = "WENO5" if rng.random() < 0.5 else "OMP6"
+        if kind == "tgv":
+            c = tgv(n, ma=ma, re=re, scheme=sch, split=1, metric_mode=METRIC_UNIFORM)
+        else:
+            c = isotropic(n, seed=20261202 + i, mt=ma, re=re, scheme=sch, split=1, char=0,
+                          metric_mode=METRIC_UNIFORM)
+        c.cfg = c.cfg.replace(precision=precision)
+        c.name = "box%d_%s%d_%s" % (i, kind, n, sch)
+        out.append(c)
+    return out

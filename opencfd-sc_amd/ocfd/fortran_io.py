@@ -77,9 +77,12 @@ def write_grid_2d_plane(path, x2, y2, z1) -> None:
             _write_record(fh, np.ascontiguousarray(a, dtype="<f8").tobytes())
 
 
-def write_grid_3d(path, X, Y, Z) -> None:
+def write_grid_3d(path, *fields) -> None:
+    """OCFD-grid.dat of Iflag_Gridtype 30 (x, y, z) or 31 (x, y, z and the ten
+    metric fields Akx .. Asz, Ajac): each field as nz records of nx*ny values
+    (read_mesh3d, OCFD_mesh.f90:188-215)."""
     with open(path, "wb") as fh:
-        for a in (X, Y, Z):
+        for a in fields:
             a = np.ascontiguousarray(a, dtype="<f8")
             for k in range(a.shape[0]):
                 _write_record(fh, a[k].tobytes())

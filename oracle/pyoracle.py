@@ -174,9 +174,10 @@ def run_probe(case, workdir, nextra=0, timeout=600, allow_negative_t=False):
 
 
 def run_reference(case, workdir, nsteps, nranks=1, decomposition=(1, 1, 1), timeout=3600,
-                  ana_box_every=0):
-    """Run the unmodified reference solver; returns (wall seconds reported by
-    the solver, final primitives)."""
+                  ana_box_every=0, exe=None):
+    """Run the reference solver (exe: REF_EXE, the unmodified one, by default;
+    REF_WALL_EXE for BC_User_Def cases that need the wall hook); returns (wall
+    seconds reported by the solver, final primitives, stdout)."""
     from ocfd import fortran_io
     os.makedirs(workdir, exist_ok=True)
     case.write_reference_inputs(workdir, nsteps=nsteps, ana_box_every=ana_box_every)
@@ -186,7 +187,7 @@ def run_reference(case, workdir, nsteps, nranks=1, decomposition=(1, 1, 1), time
         for key, val in zip(("npx0", "npy0", "npz0"), decomposition):
             s = s.replace(" %s=1\n" % key, " %s=%d\n" % (key, val))
         open(nl, "w").write(s)
-    r = subprocess.run([MPIEXEC, "-n", str(nranks), REF_EXE], cwd=workdir, capture_output=True,
+    r = subprocess.run([MPIEXEC, "-n", str(nranks), exe or REF_EXE], cwd=workdir, capture_output=True,
                        text=True, timeout=timeout)
     if r.returncode != 0:
         raise OcfdError("reference failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))
@@ -199,6 +200,19 @@ def run_reference(case, workdir, nsteps, nranks=1, decomposition=(1, 1, 1), time
     if os.path.exists(fn):
         _, _, prims = fortran_io.read_flow_data(fn, case.cfg.shape)
     return wall, prims, r.stdout
+
+
+def read_statistics(path):
+    """statistics.dat as the reference's ana_box appends it (OCFD_ANA.f90:305-308):
+    one row of 20 values per call (E20.13 each), in the column order of
+    ocfd_ana_box's output."""
+    rows = []
+    for line in open(path):
+        line = line.rstrip("\n")
+        if not line.strip():
+            continue
+        rows.append([float(line[20 * q:20 * (q + 1)].replace("D", "E")) for q in range(20)])
+    return np.array(rows)
 
 
 def count_ops(block, fn):
