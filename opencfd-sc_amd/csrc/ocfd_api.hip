@@ -62,10 +62,13 @@ struct ocfd_handle {
   BLArgs bl = {};                  // boundary-layer / swept-corner data (ibc 2, 3)
   bool bl_set = false;
   std::vector<void *> bl_mem;      // its device arrays
-  // one time step (three stages), captured on first use; [p] starts with the
-  // temperature buffers in parity p (tpar), which every stage flips
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // one time step (three stages), captured on first use; [tpar + 2 uvpar]
+  // starts with the temperature buffers in parity tpar, which every stage
+  // flips, and the u, v, w sets in parity uvpar, which every fused stage flips
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   int tpar = 0;
+  int uvpar = 0;
+  bool fused = false;              // stages run k_visc_fused (Block::u2, v2, w2 allocated)
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
@@ -438,6 +441,14 @@ void swap_T(ocfd_handle *h) {
   h->tpar ^= 1;
 }
 
+// after every fused stage: its new u, v, w (in u2, v2, w2) become current
+void swap_uvw(ocfd_handle *h) {
+  std::swap(h->B.u, h->B.u2);
+  std::swap(h->B.v, h->B.v2);
+  std::swap(h->B.w, h->B.w2);
+  h->uvpar ^= 1;
+}
+
 int read_neg(ocfd_handle *h, int out[3]) {
   HIPCHK(hipMemcpyAsync(out, h->neg, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemcpyAsync(out + 1, h->neg + 65, 2 * sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -475,8 +486,16 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   int rc = 0;
   // the primitives are FP32 in the mixed-precision form
   const bool p32 = B.prec == OCFD_PREC_MIXED_CODE;
-  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) p->p = nullptr;
+  for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2, &B.u2, &B.v2, &B.w2}) p->p = nullptr;
   for (PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) rc |= alloc_prim(p, B.g.nh, p32);
+  // the viscous terms fused into the stage update where the block allows it
+  // (OCFD_NO_FUSED keeps the two kernels, for comparisons)
+  B.M.uniform = cfg->metric_mode == OCFD_METRIC_UNIFORM;
+  h->fused = cfg->if_viscous == 1 && cfg->ibc == 0 && visc_fused_supported(B) &&
+             std::getenv("OCFD_NO_FUSED") == nullptr;
+  for (int q = 0; q < 6; q++) h->fused = h->fused && cfg->ghost_cell[q] == 0;
+  if (h->fused)
+    for (PrimField *p : {&B.u2, &B.v2, &B.w2}) rc |= alloc_prim(p, B.g.nh, p32);
   B.halt = nullptr;
   rc |= alloc(&B.f, fb) | alloc(&B.fn, fb) | alloc(&B.du, fb);
   if (cfg->scheme_invis == SCH_HYBRID) rc |= alloc(&B.rhyb, hb);
@@ -521,6 +540,8 @@ int ocfd_create(const ocfd_config *cfg, int device, ocfd_handle **out) {
   }
   // allocate_flow_data initial values (OCFD_NS_Solver.f90:116-118, 140-147)
   for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2}) (void)fill(*p, B.g.nh, 1.0, h->st);
+  if (h->fused)
+    for (const PrimField *p : {&B.u2, &B.v2, &B.w2}) (void)fill(*p, B.g.nh, 1.0, h->st);
   if (!B.M.uniform)
     for (int m = 0; m < M_N; m++) (void)fill(h->met[m], B.g.nh, 1.0, h->st);
   if (B.rhyb) (void)fill(B.rhyb, B.g.nh, 0.0, h->st);
@@ -555,7 +576,7 @@ int ocfd_destroy(ocfd_handle *h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   invalidate_graph(h);
   Block &B = h->B;
-  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2})
+  for (const PrimField *p : {&B.d, &B.u, &B.v, &B.w, &B.T, &B.T2, &B.u2, &B.v2, &B.w2})
     if (p->p) (void)hipFree(p->p);
   double *ps[] = {B.f, B.fn, B.du, B.rhyb, B.ev, B.trash, h->fbuf, h->work, h->wout};
   for (double *p : ps)
@@ -974,13 +995,21 @@ int step_body(ocfd_handle *h) {
     if (exchange_prims(h)) return -1;
     if (krk == 1 && hyb && sensor(h)) return -1;
     if (inviscid(h, ch)) return -1;
-    if (c.if_viscous == 1 && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
-    Block B = h->B;
-    if (c.if_viscous != 1) B.ev = nullptr;
     const int refresh = h->ibc == 0 ? 1 : 0;
-    if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
-      return -1;
-    if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
+    if (h->fused) {
+      // viscous terms, divergence and update in one march (k_visc_fused)
+      if (launch(h, "stage_update", [&] { return launch_visc_fused(h->B, krk, refresh, h->neg, h->st); }))
+        return -1;
+      swap_uvw(h);
+    } else {
+      if (c.if_viscous == 1 && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
+      Block B = h->B;
+      if (c.if_viscous != 1) B.ev = nullptr;
+      if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
+        return -1;
+    }
+    const int uvw = h->fused ? 1 : 0;
+    if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st, uvw); })) return -1;
     swap_T(h);
     if (h->ibc != 0) {
       if ((h->ibc == 2 || h->ibc == 3) && !h->bl_set)
@@ -999,7 +1028,7 @@ int step_body(ocfd_handle *h) {
 // host synchronisation happens inside it).
 int capture_step(ocfd_handle *h) {
   hipGraph_t graph = nullptr;
-  hipGraphExec_t *slot = &h->gexec[h->tpar];  // step_body flips tpar
+  hipGraphExec_t *slot = &h->gexec[h->tpar + 2 * h->uvpar];  // step_body flips the parities
   HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
   const int rc = step_body(h);
   const hipError_t e = hipStreamEndCapture(h->st, &graph);
@@ -1038,11 +1067,13 @@ int ocfd_step(ocfd_handle *h, int32_t nsteps) {
   int rc = 0;
   for (int n = 0; n < nsteps && !rc; n++) {
     if (graph) {
-      const int p = h->tpar;
+      const int p = h->tpar + 2 * h->uvpar;
       if (!h->gexec[p]) {
-        rc = capture_step(h);  // records the three stages, flipping tpar
+        rc = capture_step(h);  // records the three stages, flipping the parities
       } else {
-        swap_T(h);  // what the replayed stages do to the buffers
+        // what the three replayed stages do to the buffers
+        swap_T(h);
+        if (h->fused) swap_uvw(h);
       }
       if (!rc && hipGraphLaunch(h->gexec[p], h->st) != hipSuccess) rc = fail("hipGraphLaunch failed");
     } else {

@@ -47,6 +47,11 @@ struct Block {
   // reads both, copies T's halo into T2, and the host swaps the two after
   // every stage (no extra HBM traffic per cell).
   PrimField T2;
+  // The second set of u, v, w of the fused viscous stage update
+  // (k_visc_fused, mixed-precision periodic blocks; p == nullptr elsewhere):
+  // it writes the new values there while other workgroups still read the
+  // stage-start ones, and the host swaps the sets after every fused stage.
+  PrimField u2, v2, w2;
   double *f, *fn, *du;        // 5 planes each
   double *rhyb;               // halo'd sensor (hybrid scheme only)
   double *ev;                 // 12 interior planes: Ev1(1:4), Ev2(1:4), Ev3(1:4)
@@ -75,7 +80,8 @@ struct Block {
 inline bool prims_match(const Block &B) {
   const int f32 = B.prec == OCFD_PREC_MIXED_CODE;
   return B.d.f32 == f32 && B.u.f32 == f32 && B.v.f32 == f32 && B.w.f32 == f32 && B.T.f32 == f32 &&
-         B.T2.f32 == f32;
+         B.T2.f32 == f32 && (!B.u2.p || B.u2.f32 == f32) && (!B.v2.p || B.v2.f32 == f32) &&
+         (!B.w2.p || B.w2.f32 == f32);
 }
 __device__ inline bool halted(const Block &B) { return B.halt != nullptr && *B.halt != 0; }
 
@@ -269,7 +275,13 @@ hipError_t launch_copy_f(const Block &B, hipStream_t st);
 hipError_t launch_duvwt(const Block &B, int *neg, hipStream_t st);
 // T <= 0 repair of the cells listed in neg (T2 from T2 / T neighbours) and
 // the copy of T's halo into T2, in one launch
-hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st);
+// (uvw: also copy the halos of u2, v2, w2 into u, v, w -- after a fused
+// stage, whose new u, v, w the host has swapped in)
+hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st, int uvw = 0);
+// the viscous terms fused into the stage update (k_visc_fused): whether this
+// block can take it, and the launch (writes u2, v2, w2 and T2)
+bool visc_fused_supported(const Block &B);
+hipError_t launch_visc_fused(const Block &B, int krk, int refresh, int *neg, hipStream_t st);
 // workgroups of the show_flow_msg reduction (3 partial sums each)
 int64_t flow_msg_partials(const Geom &g);
 // largest CFL rate over the cells, as the bit pattern of a positive double,

@@ -347,7 +347,7 @@ __device__ void negt_repair_body(const Block &B, int *neg, const int refresh) {
 // of the halo'd array: rows outside the interior j, k range are copied whole,
 // the others at their 2 * LAP i-halo points.
 constexpr int HCOPY_ROWS = 16;
-__device__ inline void halo_copy_tile(const Block &B, const int64_t q) {
+__device__ inline void halo_copy_tile(const Block &B, const int64_t q, const int uvw) {
   const Geom &g = B.g;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t nrows = (int64_t)g.sy * g.sz;
@@ -362,20 +362,33 @@ __device__ inline void halo_copy_tile(const Block &B, const int64_t q) {
     for (int p = lane; p < npts; p += 64) {
       const int ih = whole ? p : (p < LAP ? p : g.nx + p);
       const int64_t x = base + ih;
-      if (f32) B.T2.as<float>()[x] = B.T.as<float>()[x];
-      else B.T2.as<double>()[x] = B.T.as<double>()[x];
+      if (f32) {
+        B.T2.as<float>()[x] = B.T.as<float>()[x];
+        if (uvw) {
+          B.u.as<float>()[x] = B.u2.as<float>()[x];
+          B.v.as<float>()[x] = B.v2.as<float>()[x];
+          B.w.as<float>()[x] = B.w2.as<float>()[x];
+        }
+      } else {
+        B.T2.as<double>()[x] = B.T.as<double>()[x];
+        if (uvw) {
+          B.u.as<double>()[x] = B.u2.as<double>()[x];
+          B.v.as<double>()[x] = B.v2.as<double>()[x];
+          B.w.as<double>()[x] = B.w2.as<double>()[x];
+        }
+      }
     }
   }
 }
 inline int64_t halo_copy_tiles_of(const Geom &g) { return ((int64_t)g.sy * g.sz + HCOPY_ROWS - 1) / HCOPY_ROWS; }
 
 // workgroup 0: the repair; workgroups 1..: the halo copy
-__global__ __launch_bounds__(NT) void k_negt_repair(const Block B, int *neg, const int refresh) {
+__global__ __launch_bounds__(NT) void k_negt_repair(const Block B, int *neg, const int refresh, const int uvw) {
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) negt_repair_body(B, neg, refresh);
     return;
   }
-  halo_copy_tile(B, blockIdx.x - 1);
+  halo_copy_tile(B, blockIdx.x - 1, uvw);
 }
 
 // batches: workgroup b < nbox repairs box b (its counters at neg + NEG_STRIDE
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_negt_repair_batch(const BoxSet S, const 
   }
   const int t = blockIdx.x - nbox;
   const int j = S.entry(t);
-  halo_copy_tile(S.blk[S.ids[j]], t - S.off[j]);
+  halo_copy_tile(S.blk[S.ids[j]], t - S.off[j], 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1065,8 +1078,9 @@ hipError_t launch_cfl_rate(const Block &B, unsigned long long *out, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st) {
-  hipLaunchKernelGGL(k_negt_repair, dim3((unsigned)(1 + halo_copy_tiles_of(B.g))), dim3(NT), 0, st, B, neg, 1);
+hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st, int uvw) {
+  if (uvw && (!B.u2.p || !B.v2.p || !B.w2.p)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_negt_repair, dim3((unsigned)(1 + halo_copy_tiles_of(B.g))), dim3(NT), 0, st, B, neg, 1, uvw);
   return hipGetLastError();
 }
 

@@ -102,6 +102,61 @@ __device__ inline int wrap_index(int t, int n, int R) {
   return t < 1 ? t + n : (t > n ? t - n : t);
 }
 
+// The viscous fluxes Ev1(1:4), Ev2(1:4), Ev3(1:4) of one point
+// (OCFD_viscous.f90:42-111, comput_Amu :135-149) from the derivatives of
+// u, v, w, T along the three grid lines (dx, dy, dz), the point's u, v, w,
+// T and the metrics at halo'd index h (constants when FAST).
+template <bool FAST, typename RT>
+__device__ __forceinline__ void ev_point(const Block &B, const RT *dx, const RT *dy, const RT *dz, const RT uu,
+                                         const RT vv, const RT ww, const RT Tc, const int64_t h, RT *out) {
+  const Metrics &M = B.M;
+  auto met = [&](int qm) { return RT(FAST ? M.u[qm] : M.get(qm, h)); };
+  const RT Tsb = RT(110.4 / B.P.ref_amu_t0), TmpR = RT(1.0 / B.P.re);
+  const RT cp = RT(B.P.cp), pr = RT(B.P.pr);
+  // comput_Amu (OCFD_viscous.f90:135-149)
+  const RT amu = TmpR * (RT(1.0) + Tsb) * sqrt(Tc * Tc * Tc) / (Tsb + Tc);
+  // OCFD_viscous.f90:42-53
+  const RT uk = dx[0], ui = dy[0], us = dz[0];
+  const RT vk = dx[1], vi = dy[1], vs = dz[1];
+  const RT wk = dx[2], wi = dy[2], ws = dz[2];
+  const RT Tk = dx[3], Ti = dy[3], Ts = dz[3];
+  const RT akx = met(M_AKX), aky = met(M_AKY), akz = met(M_AKZ);
+  const RT aix = met(M_AIX), aiy = met(M_AIY), aiz = met(M_AIZ);
+  const RT asx = met(M_ASX), asy = met(M_ASY), asz = met(M_ASZ);
+  // OCFD_viscous.f90:60-111
+  const RT ux = uk * akx + ui * aix + us * asx;
+  const RT vx = vk * akx + vi * aix + vs * asx;
+  const RT wx = wk * akx + wi * aix + ws * asx;
+  const RT Tx = Tk * akx + Ti * aix + Ts * asx;
+  const RT uy = uk * aky + ui * aiy + us * asy;
+  const RT vy = vk * aky + vi * aiy + vs * asy;
+  const RT wy = wk * aky + wi * aiy + ws * asy;
+  const RT Ty = Tk * aky + Ti * aiy + Ts * asy;
+  const RT uz = uk * akz + ui * aiz + us * asz;
+  const RT vz = vk * akz + vi * aiz + vs * asz;
+  const RT wz = wk * akz + wi * aiz + ws * asz;
+  const RT Tz = Tk * akz + Ti * aiz + Ts * asz;
+  const RT div = ux + vy + wz;
+  const RT Amu1 = amu;
+  const RT Amuk = cp * amu / pr;
+  const RT tmp2_3 = RT(2.0 / 3.0);
+  const RT s11 = (RT(2.0) * ux - tmp2_3 * div) * Amu1;
+  const RT s22 = (RT(2.0) * vy - tmp2_3 * div) * Amu1;
+  const RT s33 = (RT(2.0) * wz - tmp2_3 * div) * Amu1;
+  const RT s12 = (uy + vx) * Amu1, s13 = (uz + wx) * Amu1, s23 = (vz + wy) * Amu1;
+  const RT E1 = uu * s11 + vv * s12 + ww * s13 + Amuk * Tx;
+  const RT E2 = uu * s12 + vv * s22 + ww * s23 + Amuk * Ty;
+  const RT E3 = uu * s13 + vv * s23 + ww * s33 + Amuk * Tz;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const RT n1 = met(M_AKX1 + 3 * a), n2 = met(M_AKX1 + 3 * a + 1), n3 = met(M_AKX1 + 3 * a + 2);
+    out[4 * a + 0] = n1 * s11 + n2 * s12 + n3 * s13;
+    out[4 * a + 1] = n1 * s12 + n2 * s22 + n3 * s23;
+    out[4 * a + 2] = n1 * s13 + n2 * s23 + n3 * s33;
+    out[4 * a + 3] = n1 * E1 + n2 * E2 + n3 * E3;
+  }
+}
+
 // Lanes whose column lies outside the block (partial tiles only) run the
 // same code on clamped indices and store into Block::trash, and the tile
 // loads and stores clamp their element index, so the plane loop has no
@@ -162,10 +217,6 @@ __device__ __forceinline__ void visc_flux_tile(const Block &B, const int kseg, c
   store_tile(sT[0]);
   __syncthreads();
 
-  const Metrics &M = B.M;
-  auto met = [&](int qm, int64_t h) { return RT(FAST ? M.u[qm] : M.get(qm, h)); };
-  const RT Tsb = RT(110.4 / B.P.ref_amu_t0), TmpR = RT(1.0 / B.P.re);
-  const RT cp = RT(B.P.cp), pr = RT(B.P.pr);
   RT *const ev = reinterpret_cast<RT *>(B.ev);
   RT *const trash = reinterpret_cast<RT *>(B.trash);
   const bool perx = FAST || B.S.iperiodic[0] == 1, pery = FAST || B.S.iperiodic[1] == 1,
@@ -194,54 +245,13 @@ __device__ __forceinline__ void visc_flux_tile(const Block &B, const int kseg, c
       // one field's stencil values in flight at a time
       if (FAST) __builtin_amdgcn_sched_barrier(0);
     }
-    // comput_Amu (OCFD_viscous.f90:135-149)
-    const RT Tc = q[3][R];
-    const RT amu = TmpR * (RT(1.0) + Tsb) * sqrt(Tc * Tc * Tc) / (Tsb + Tc);
-    // OCFD_viscous.f90:42-53
-    const RT uk = dx[0], ui = dy[0], us = dz[0];
-    const RT vk = dx[1], vi = dy[1], vs = dz[1];
-    const RT wk = dx[2], wi = dy[2], ws = dz[2];
-    const RT Tk = dx[3], Ti = dy[3], Ts = dz[3];
-    const RT akx = met(M_AKX, h), aky = met(M_AKY, h), akz = met(M_AKZ, h);
-    const RT aix = met(M_AIX, h), aiy = met(M_AIY, h), aiz = met(M_AIZ, h);
-    const RT asx = met(M_ASX, h), asy = met(M_ASY, h), asz = met(M_ASZ, h);
-    // OCFD_viscous.f90:60-111
-    const RT ux = uk * akx + ui * aix + us * asx;
-    const RT vx = vk * akx + vi * aix + vs * asx;
-    const RT wx = wk * akx + wi * aix + ws * asx;
-    const RT Tx = Tk * akx + Ti * aix + Ts * asx;
-    const RT uy = uk * aky + ui * aiy + us * asy;
-    const RT vy = vk * aky + vi * aiy + vs * asy;
-    const RT wy = wk * aky + wi * aiy + ws * asy;
-    const RT Ty = Tk * aky + Ti * aiy + Ts * asy;
-    const RT uz = uk * akz + ui * aiz + us * asz;
-    const RT vz = vk * akz + vi * aiz + vs * asz;
-    const RT wz = wk * akz + wi * aiz + ws * asz;
-    const RT Tz = Tk * akz + Ti * aiz + Ts * asz;
-    const RT div = ux + vy + wz;
-    const RT Amu1 = amu;
-    const RT Amuk = cp * amu / pr;
-    const RT tmp2_3 = RT(2.0 / 3.0);
-    const RT s11 = (RT(2.0) * ux - tmp2_3 * div) * Amu1;
-    const RT s22 = (RT(2.0) * vy - tmp2_3 * div) * Amu1;
-    const RT s33 = (RT(2.0) * wz - tmp2_3 * div) * Amu1;
-    const RT s12 = (uy + vx) * Amu1, s13 = (uz + wx) * Amu1, s23 = (vz + wy) * Amu1;
-    const RT uu = q[0][R], vv = q[1][R], ww = q[2][R];
-    const RT E1 = uu * s11 + vv * s12 + ww * s13 + Amuk * Tx;
-    const RT E2 = uu * s12 + vv * s22 + ww * s23 + Amuk * Ty;
-    const RT E3 = uu * s13 + vv * s23 + ww * s33 + Amuk * Tz;
+    RT e12[12];
+    ev_point<FAST, RT>(B, dx, dy, dz, q[0][R], q[1][R], q[2][R], q[3][R], h, e12);
     // Ev is stored non-temporally: the L2 keeps the primitive planes that
     // the neighbouring tiles read as their halos
     RT *eb = tc.valid ? ev + g.ii(i, j, k) : trash + threadIdx.x;
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-      const RT n1 = met(M_AKX1 + 3 * a, h), n2 = met(M_AKX1 + 3 * a + 1, h), n3 = met(M_AKX1 + 3 * a + 2, h);
-      RT *e = eb + (int64_t)(4 * a) * es;
-      __builtin_nontemporal_store(n1 * s11 + n2 * s12 + n3 * s13, &e[0]);
-      __builtin_nontemporal_store(n1 * s12 + n2 * s22 + n3 * s23, &e[es]);
-      __builtin_nontemporal_store(n1 * s13 + n2 * s23 + n3 * s33, &e[2 * es]);
-      __builtin_nontemporal_store(n1 * E1 + n2 * E2 + n3 * E3, &e[3 * es]);
-    }
+    for (int c = 0; c < 12; c++) __builtin_nontemporal_store(e12[c], &eb[(int64_t)c * es]);
     // plane k+1 into the other buffer (its last readers, plane k-1, are past
     // the previous barrier); the queue moves up one plane
     store_tile(sT[(k + 1 - tc.kb) & 1]);

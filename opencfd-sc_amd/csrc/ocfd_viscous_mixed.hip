@@ -2,6 +2,7 @@
 // the mixed-precision form (Ev, the derivatives, stresses and fluxes in FP32;
 // du accumulated in FP32; the stage combination and primitives FP64).
 #include "ocfd_visc.h"
+#include "ocfd_visc_fused.h"
 
 namespace ocfd {
 
@@ -11,6 +12,10 @@ hipError_t launch_visc_div_mixed(const Block &B, hipStream_t st) { return visc_d
 
 hipError_t launch_visc_div_update_mixed(const Block &B, int krk, int refresh_planes, int *neg, hipStream_t st) {
   return visc_div_update_as<float>(B, krk, refresh_planes, neg, st);
+}
+
+hipError_t launch_visc_fused_mixed(const Block &B, int krk, int refresh, int *neg, hipStream_t st) {
+  return visc_fused_as<float>(B, krk, refresh, neg, st);
 }
 
 hipError_t launch_visc_flux_batch_mixed(const BoxSet &S, bool cd8, int ntiles, hipStream_t st) {
