@@ -183,7 +183,8 @@ def hybrid_mix(b):
 # ---------------------------------------------------------------------------
 PROFILE_FILE = {("iso512", "fp64"): "stage_update_iso512_fp64.md",
                 ("iso512", "mixed"): "stage_update_iso512_mixed.md"}
-KERNEL_SYMBOL = {"stage_update": "k_visc_div_update", "visc_flux": "k_visc_flux", "halo_fill": "k_halo",
+KERNEL_SYMBOL = {"stage_update": "k_visc_div_update", "stage_update_fused": "k_visc_fused",
+                 "visc_flux": "k_visc_flux", "halo_fill": "k_halo",
                  "inviscid_x": "k_inviscid<0", "inviscid_y": "k_inviscid<1", "inviscid_z": "k_inviscid<2",
                  "inviscid_char_x": "k_inviscid<0", "inviscid_char_y": "k_inviscid<1",
                  "inviscid_char_z": "k_inviscid<2", "weno_faces": "k_weno_faces", "sensor": "k_sensor1",
@@ -236,7 +237,7 @@ def kernel_roofline(name, ms, cnt, cfg, variant, mix, prec, workload):
                 "note": "no MFMA: the stencils have no dense contraction; %s vector rate; exact operation "
                         "counts of the reference algorithm" % ("FP64" if prec == PREC_FP64 else "FP32")}
     else:
-        if name == "stage_update":
+        if name in ("stage_update", "stage_update_fused"):
             byts = float(perfmodel.COMPULSORY_STAGE_BYTES * cells)
             basis = "120 B per cell and stage: 5 conserved variables read, 5 stage-start values read, 5 written (FP64)"
         else:

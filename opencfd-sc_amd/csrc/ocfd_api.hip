@@ -998,7 +998,7 @@ int step_body(ocfd_handle *h) {
     const int refresh = h->ibc == 0 ? 1 : 0;
     if (h->fused) {
       // viscous terms, divergence and update in one march (k_visc_fused)
-      if (launch(h, "stage_update", [&] { return launch_visc_fused(h->B, krk, refresh, h->neg, h->st); }))
+      if (launch(h, "stage_update_fused", [&] { return launch_visc_fused(h->B, krk, refresh, h->neg, h->st); }))
         return -1;
       swap_uvw(h);
     } else {
