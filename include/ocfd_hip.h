@@ -222,9 +222,11 @@ int ocfd_step(ocfd_handle *h, int32_t nsteps);
 
 /* ---- sweep tiles (no counterpart in the reference) --------------------- */
 /* Tile shape of the inviscid sweep of each axis: 0 = x 64x4, y/z 32x8 points
- * x lines (the default); 1 = x 32x8, y/z 16x16; 2 = x 128x2, y/z 64x4.
- * Shapes 1 and 2 apply to periodic axes with OCFD_METRIC_UNIFORM; other axes
- * always use 0.  Results are bit-identical for every shape. */
+ * x lines (the default); 1 = x 32x8, y/z 16x16; 2 = x 128x2, y/z 64x4;
+ * 3 and 4 (y and z only) march along the axis, one lane per line, over
+ * segments of 32 and 64 points, keeping the split fluxes in registers.
+ * Shapes 1-4 apply to periodic axes with OCFD_METRIC_UNIFORM whose extents
+ * fit them; other axes use 0.  Results are bit-identical for every shape. */
 int ocfd_set_sweep_tiles(ocfd_handle *h, const int32_t shape[3]);
 int ocfd_get_sweep_tiles(ocfd_handle *h, int32_t shape[3]);
 /* Autotuner: times `reps` launches of each axis' sweep in every shape it
