@@ -99,6 +99,51 @@ def build_case(workload: str, n_override=None):
     raise SystemExit("unknown workload %s" % workload)
 
 
+DROPIN_N = 256
+DROPIN_STEPS = 20
+
+
+def dropin_leg(n=DROPIN_N, steps=DROPIN_STEPS):
+    """The reference's own driver with the hot path served by this library
+    (opencfd-sc_amd/fortran: NS_solver, one MPI rank, state on the device)
+    against ocfd_step on the same n^3 isotropic case: hybrid characteristic
+    LLF in FP64 with the field metrics the reference's init_mesh computes
+    (the drop-in's metric form).  The drop-in runs 2 x `steps` steps and is
+    timed by the reference's own show_flow_msg timer over the second block
+    of `steps`, once with stage fusion (its default) and once with one kernel
+    sequence per routine."""
+    import shutil
+    from ocfd import dropin
+    if not dropin.available():
+        return {"skipped": "drop-in executable or mpiexec missing"}
+    case = cases.isotropic(n, scheme="HYBRID", split=2, char=1, hybrid_thresholds=HYB_THRESHOLDS,
+                           metric_mode=METRIC_FIELD)
+    cells = case.cfg.nx * case.cfg.ny * case.cfg.nz
+    with native.setup_block(native.Block(case.cfg), case) as b:
+        b.step(steps)
+        b.synchronize()
+        t = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        lib_s = time.perf_counter() - t
+    out = {"workload": "decaying compressible isotropic turbulence %d^3, hybrid characteristic LLF, FP64, "
+                       "field metrics, initial field" % n,
+           "steps_timed": steps, "ocfd_step": round(cells * steps / lib_s / 1e6, 3)}
+    log("drop-in leg: ocfd_step %.1f Mcell-RK3-steps/s on %d^3 field metrics" % (out["ocfd_step"], n))
+    for key, env in (("dropin", {}), ("dropin_per_routine_kernels", {"OCFD_HIP_STAGE_FUSION": "0"})):
+        wd = tempfile.mkdtemp(prefix="ocfd_dropin_")
+        try:
+            t = time.time()
+            _, iv, total = dropin.run(case, wd, 2 * steps, show_every=steps, env=env)
+            out[key] = round(cells * steps / iv[-1] / 1e6, 3)
+            log("drop-in leg: %s %.1f Mcell-RK3-steps/s (%.3f s per %d steps; run %.1f s)" % (
+                key, out[key], iv[-1], steps, time.time() - t))
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+    out["dropin_over_ocfd_step"] = round(out["dropin"] / out["ocfd_step"], 4)
+    return out
+
+
 def spinup_steps_for(workload: str, case) -> int:
     """Steps that take the isotropic field to the end time of the 256^3 run
     (200 steps; dt scales with the grid spacing, so 400 at 512^3)."""
@@ -471,6 +516,9 @@ def main():
     ap.add_argument("--other-form", type=int, default=1,
                     help="also run the other precision form on the same workload and report it (default 1)")
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
+    ap.add_argument("--dropin", type=int, default=1,
+                    help="also time the reference driver with this library as its hot path "
+                         "(%d^3, %d steps; default 1)" % (DROPIN_N, DROPIN_STEPS))
     ap.add_argument("--trace-marker", type=int, default=0,
                     help="launch the CFL reduction kernel once right before the timed steps, so that a "
                          "rocprofv3 trace or counter run can keep only the dispatches after it")
@@ -567,6 +615,8 @@ def main():
     for r in runs.values():
         r["b"].close()
     runs.clear()
+    if args.dropin:
+        res["dropin"] = dropin_leg()
     res["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_baseline)
     print(json.dumps(res), flush=True)
 

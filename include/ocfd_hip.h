@@ -178,6 +178,25 @@ int ocfd_save_fn(ocfd_handle *h);
 int ocfd_comput_duvwt(ocfd_handle *h);
 /* OCFD_bc (OCFD_BC.f90:2-99) incl. Flow_Ghost_boundary. */
 int ocfd_bc(ocfd_handle *h);
+/* Stage fusion (no counterpart in the reference, whose routines each sweep
+ * the whole block).  For a caller that issues the routines above in
+ * NS_solver's order each stage (OCFD_NS_Solver.f90:33-60: exchange,
+ * comput_Rhybrid, du_inviscous[_Character], du_viscous, OCFD_time_adv_RK3,
+ * comput_duvwT, OCFD_bc), on = 1 merges them into ocfd_step's stage
+ * kernels, with the same results bit for bit:
+ *   ocfd_du_viscous      forms only the viscous fluxes;
+ *   ocfd_time_adv_rk3    adds their divergence, applies RK3, stores fn = f
+ *                        at KRK = 1, recovers the primitives, refreshes f on
+ *                        the boundary planes of a periodic block (ibc 0) and
+ *                        lists the T <= 0 cells, in one kernel;
+ *   ocfd_save_fn         does nothing (ocfd_time_adv_rk3(1) stores fn);
+ *   ocfd_comput_duvwt    repairs the listed cells and returns their count;
+ *   ocfd_bc              on a periodic block, only the ghost cells.
+ * Arrays read between the calls of one stage differ from the per-routine
+ * path's: after ocfd_du_viscous du holds no viscous terms, and after
+ * ocfd_time_adv_rk3 f is already refreshed on the boundary planes.  on = 0
+ * (the default) restores one kernel sequence per routine. */
+int ocfd_set_stage_fusion(ocfd_handle *h, int32_t on);
 /* Data of BoundaryCondition = BC_BoundaryLayer (ibc 2, ocfd_bc_BoundaryLayer,
  * OCFD_BC_Boundarylayers.f90:17-167) or BC_SweptCorner (ibc 3,
  * ocfd_bc_swept_corner, :443-617), which the reference prepares on the host

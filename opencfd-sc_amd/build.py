@@ -25,7 +25,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["ocfd_inviscid.hip", "ocfd_inviscid_mixed.hip", "ocfd_hybrid.hip", "ocfd_viscous.hip",
-           "ocfd_viscous_mixed.hip", "ocfd_misc.hip", "ocfd_api.hip"]
+           "ocfd_viscous_mixed.hip", "ocfd_misc.hip", "ocfd_batchq.hip", "ocfd_batchq_mixed.hip",
+           "ocfd_api.hip"]
 
 FLAGS = [
     "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

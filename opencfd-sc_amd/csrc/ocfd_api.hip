@@ -69,6 +69,10 @@ struct ocfd_handle {
   int tpar = 0;
   int uvpar = 0;
   bool fused = false;              // stages run k_visc_fused (Block::u2, v2, w2 allocated)
+  // per-routine calls merged into the whole-step stage kernels
+  // (ocfd_set_stage_fusion); pend holds what the stage so far left to do
+  bool stage_fusion = false;
+  int pend = 0;
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
@@ -448,6 +452,9 @@ void swap_uvw(ocfd_handle *h) {
   std::swap(h->B.w, h->B.w2);
   h->uvpar ^= 1;
 }
+
+// ocfd_handle::pend bits (stage fusion)
+constexpr int PEND_VISC = 1, PEND_UPDATE = 2, PEND_REFRESHED = 4, PEND_UVW = 8;
 
 int read_neg(ocfd_handle *h, int out[3]) {
   HIPCHK(hipMemcpyAsync(out, h->neg, sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -859,19 +866,68 @@ int ocfd_du_inviscous_character(ocfd_handle *h) { return inviscid(h, true); }
 
 int ocfd_du_viscous(ocfd_handle *h) {
   if (!h->B.ev) return fail("du_viscous needs IF_Viscous = 1");
+  if (h->stage_fusion) {
+    // the fluxes now (inside the update for a fused block); their
+    // divergence joins the update in ocfd_time_adv_rk3
+    if (!h->fused && launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
+    h->pend |= PEND_VISC;
+    return 0;
+  }
   if (launch(h, "visc_flux", [&] { return launch_visc_flux(h->B, h->st); })) return -1;
   return launch(h, "visc_div", [&] { return launch_visc_div(h->B, h->st); });
 }
 
 int ocfd_time_adv_rk3(ocfd_handle *h, int32_t krk) {
   if (krk < 1 || krk > 3) return fail("KRK must be 1, 2 or 3");
+  if (h->stage_fusion) {
+    // ocfd_step's stage update: viscous divergence (when du_viscous ran),
+    // RK3 (fn = f at KRK = 1), primitives, boundary-plane f on periodic
+    // blocks, and the list of T <= 0 cells for ocfd_comput_duvwt
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, NEG_STRIDE);
+    const int refresh = h->ibc == 0 ? 1 : 0;
+    const bool visc = (h->pend & PEND_VISC) != 0;
+    int pend = PEND_UPDATE | (refresh ? PEND_REFRESHED : 0);
+    if (visc && h->fused) {
+      if (launch(h, "stage_update_fused", [&] { return launch_visc_fused(h->B, krk, refresh, h->neg, h->st); }))
+        return -1;
+      swap_uvw(h);
+      pend |= PEND_UVW;
+    } else {
+      Block B = h->B;
+      if (!visc) B.ev = nullptr;
+      if (launch(h, "stage_update", [&] { return launch_visc_div_update(B, krk, refresh, h->neg, h->st); }))
+        return -1;
+    }
+    h->pend = pend;
+    return 0;
+  }
   return launch(h, "rk3", [&] { return launch_rk3(h->B, krk, h->st); });
 }
 
-int ocfd_save_fn(ocfd_handle *h) { return launch(h, "copy_fn", [&] { return launch_copy_f(h->B, h->st); }); }
+int ocfd_save_fn(ocfd_handle *h) {
+  if (h->stage_fusion) return 0;  // ocfd_time_adv_rk3(1) stores fn = f
+  return launch(h, "copy_fn", [&] { return launch_copy_f(h->B, h->st); });
+}
+
+int ocfd_set_stage_fusion(ocfd_handle *h, int32_t on) {
+  h->stage_fusion = on != 0;
+  h->pend = 0;
+  return 0;
+}
 
 int ocfd_comput_duvwt(ocfd_handle *h) {
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, 72);
+  if (h->stage_fusion && (h->pend & PEND_UPDATE)) {
+    // primitives already recovered by the stage update: repair and count
+    const int uvw = (h->pend & PEND_UVW) ? 1 : 0;
+    if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st, uvw); })) return -1;
+    swap_T(h);
+    h->pend &= ~(PEND_UPDATE | PEND_UVW);
+    int c[3];
+    if (read_neg(h, c)) return -1;
+    return c[1];
+  }
+  h->pend = 0;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, NEG_STRIDE);
   if (launch(h, "duvwt", [&] { return launch_duvwt(h->B, h->neg, h->st); })) return -1;
   if (launch(h, "negt_repair", [&] { return launch_negt_repair(h->B, h->neg, h->st); })) return -1;
   swap_T(h);
@@ -883,6 +939,15 @@ int ocfd_comput_duvwt(ocfd_handle *h) {
 int ocfd_bc(ocfd_handle *h) {
   if ((h->ibc == 2 || h->ibc == 3) && !h->bl_set)
     return fail("BC_BoundaryLayer / BC_SweptCorner: call ocfd_set_bc_boundarylayer first");
+  const bool refreshed = h->stage_fusion && (h->pend & PEND_REFRESHED);
+  h->pend = 0;
+  if (refreshed) {
+    // the stage update refreshed f on the boundary planes: ghost cells only
+    bool any = false;
+    for (int q = 0; q < 6; q++) any |= (h->ghost[q] == 1 || h->ghost[q] == 2);
+    if (!any) return 0;
+    return launch(h, "bc", [&] { return launch_bc(h->B, -1, h->tw, nullptr, h->ghost, 0, h->st); });
+  }
   return launch(h, "bc", [&] { return launch_bc(h->B, h->ibc, h->tw, &h->bl, h->ghost, 1, h->st); });
 }
 
@@ -1055,6 +1120,7 @@ void invalidate_graph(ocfd_handle *h) {
 
 int ocfd_step(ocfd_handle *h, int32_t nsteps) {
   const ocfd_config &c = h->cfg;
+  h->pend = 0;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, h->st, h->neg, NEG_STRIDE);
   // filtering decides per step on the host; per-kernel timing needs eager launches
   const bool graph = h->use_graph && !h->prof && c.nfiltering == 0;

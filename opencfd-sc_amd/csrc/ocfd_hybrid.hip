@@ -22,7 +22,6 @@ namespace ocfd {
 
 namespace {
 
-constexpr int NT = 256;
 constexpr int FB = 51;   // listed faces per workgroup batch: 5 * FB rows fill the 256 lanes
 constexpr int SFS = 81;  // LDS stride of one face's split fluxes (8 points x 10, padded so
                          // that the faces a wavefront reads fall in different banks)

@@ -260,6 +260,33 @@ constexpr int NBATCH_KSEG = 4;
 constexpr int BATCH_KSEGS[NBATCH_KSEG] = {4, 8, 16, 32};
 // workgroups of the periodic halo fill of one box
 int64_t halo_tiles(const Geom &g);
+
+// One persistent launch per batched time step (ocfd_batch, queue mode): the
+// per-phase launches become segments of a work queue of (segment, tile)
+// items; Q_NKIND phase kinds per stage.
+enum QueueKind : int { Q_HALO = 0, Q_X, Q_Y, Q_Z, Q_VFLUX, Q_UPDATE, Q_REPAIR, Q_NKIND };
+struct QSeg {
+  int kind, stage;     // phase kind, RK stage 1..3
+  int sch, visc, cd8;  // sweeps: scheme; flux / update: viscous terms, CD8
+  int ids, off, m;     // box list and tile offsets in the batch's table (BoxSet)
+  int first, ntiles;   // first ticket, items
+};
+struct QPlan {
+  const Block *blk0, *blk1;  // descriptor tables; stage s reads table q0 ^ ((s - 1) & 1)
+  int q0;
+  const int *tab;
+  const QSeg *seg;
+  int nseg, nitems;
+  int *head;        // [0] next ticket, [1] set when a wait ran out
+  int *done;        // [box][stage][Q_NKIND] finished items
+  const int *need;  // [box][Q_NKIND] items of each kind in one stage
+  int *neg;         // T <= 0 counters, NEG_STRIDE per box
+};
+// the sweep shape (0-4) the queue kernel runs along `axis`
+int queue_shape(int axis);
+hipError_t launch_batch_queue(const QPlan &P, int prec, int nwg, hipStream_t st);
+// workgroups of the queue kernel resident at once (< 0: query failed)
+int batch_queue_grid(int prec, int device);
 hipError_t launch_weno_faces(const Block &B, int axis, hipStream_t st);
 hipError_t launch_halo_fill(const Block &B, double *const *fields, int nfields, hipStream_t st);
 // the same for primitive fields (at most 5, all of one precision)
@@ -298,5 +325,16 @@ hipError_t launch_ana_box(const Block &B, double *grad, double *amu, double *par
 hipError_t launch_metrics(const Block &B, int ibc, double *coords[3], double *met[M_N], const double spans[9],
                           const int ghost[6], double *grad, hipStream_t st);
 hipError_t launch_normalise_metrics(const Block &B, double *met[M_N], hipStream_t st);
+
+// threads per workgroup of the element-wise, viscous and stage kernels
+constexpr int NT = 256;
+
+// interior (i, j, k), 1-based, of cell t of the nx * ny * nz block (i fastest)
+__device__ inline void cell_of(const Geom &g, int64_t t, int &i, int &j, int &k) {
+  i = (int)(t % g.nx) + 1;
+  int64_t r = t / g.nx;
+  j = (int)(r % g.ny) + 1;
+  k = (int)(r / g.ny) + 1;
+}
 
 }  // namespace ocfd

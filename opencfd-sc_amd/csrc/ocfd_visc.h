@@ -26,15 +26,6 @@ namespace ocfd {
 
 namespace {
 
-constexpr int NT = 256;
-
-__device__ inline void cell_of(const Geom &g, int64_t t, int &i, int &j, int &k) {
-  i = (int)(t % g.nx) + 1;
-  int64_t r = t / g.nx;
-  j = (int)(r % g.ny) + 1;
-  k = (int)(r / g.ny) + 1;
-}
-
 // derivative of a halo'd field along an axis at interior point (i,j,k)
 __device__ inline double dh(const Block &B, const double *a, int axis, int i, int j, int k) {
   const Geom &g = B.g;

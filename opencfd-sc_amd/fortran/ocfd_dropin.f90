@@ -30,6 +30,19 @@
 !                                   case whose ocfd_bc_User is that wall may ask
 !                                   for it.
 !   filtering(f)                 -> ocfd_advance_clock + ocfd_filtering (OCFD_filtering.f90:4-70)
+!   show_flow_msg                -> ocfd_show_flow_msg (OCFD_Time_Adv.f90:65-104), printed
+!                                   and logged in the reference's formats
+!   OCFD_analysis                -> ocfd_ana_box when every analysis due is ana_box
+!                                   (OCFD_ANA.f90:178-311); any other on the host
+!
+! The stage's routines run in stage fusion (ocfd_set_stage_fusion): the
+! library merges du_viscous, OCFD_time_adv_RK3 and comput_duvwT into the
+! kernels of its whole-step path, with the same results.  Environment:
+!   OCFD_HIP_STAGE_FUSION=0  one kernel sequence per routine instead
+!   OCFD_HIP_DIAG=host       show_flow_msg / OCFD_analysis on the host after a
+!                            download (their printed sums then equal the
+!                            reference's to the last digit; the device sums
+!                            add in another order, within 1e-13)
 module ocfd_dropin_state
    use iso_c_binding
    implicit none
@@ -40,6 +53,8 @@ module ocfd_dropin_state
    ! BC_User_Def served by the library's device hook (OCFD_HIP_USER_BC=device)
    ! instead of the user's host routine
    logical, save :: user_bc_on_device = .false.
+   ! show_flow_msg / ana_box from the device sums (OCFD_HIP_DIAG /= host)
+   logical, save :: diag_on_device = .true.
 end module ocfd_dropin_state
 
 ! Creates the device block from the reference's parameters and uploads the
@@ -51,7 +66,7 @@ subroutine ocfd_dropin_attach()
    implicit none
    type(ocfd_config) :: cfg
    type(c_ptr) :: mets(19)
-   integer :: dev
+   integer :: dev, fusion
    character(len=16) :: env
    integer :: st
 
@@ -97,7 +112,21 @@ subroutine ocfd_dropin_attach()
          stop
       end if
    end if
+   diag_on_device = .true.
+   call get_environment_variable('OCFD_HIP_DIAG', env, status=st)
+   if (st == 0) then
+      if (trim(env) == 'host') then
+         diag_on_device = .false.
+      else if (trim(env) /= 'device') then
+         print *, 'libocfd_hip drop-in: OCFD_HIP_DIAG must be host or device, not ', trim(env)
+         stop
+      end if
+   end if
+   fusion = 1
+   call get_environment_variable('OCFD_HIP_STAGE_FUSION', env, status=st)
+   if (st == 0) read (env, *) fusion
    call ocfd_check(ocfd_create(cfg, int(dev, c_int), hdev), 'ocfd_create')
+   call ocfd_check(ocfd_set_stage_fusion(hdev, int(fusion, c_int32_t)), 'ocfd_set_stage_fusion')
 
    mets = [ocfd_addr(Akx), ocfd_addr(Aky), ocfd_addr(Akz), ocfd_addr(Aix), ocfd_addr(Aiy), &
            ocfd_addr(Aiz), ocfd_addr(Asx), ocfd_addr(Asy), ocfd_addr(Asz), ocfd_addr(Ajac), &
@@ -378,38 +407,100 @@ subroutine wrap_filtering(fa) bind(C, name="__wrap_filtering_")
 end subroutine wrap_filtering
 
 subroutine wrap_show_flow_msg(wall_time) bind(C, name="__wrap_show_flow_msg_")
+   use flow_data
+   use ocfd_hip
    use ocfd_dropin_state
    implicit none
    real(c_double) :: wall_time
+   real(c_double) :: e(3), t0
    interface
       subroutine real_show_flow_msg(wall_time) bind(C, name="__real_show_flow_msg_")
          import :: c_double
          real(c_double) :: wall_time
       end subroutine
    end interface
-   call ocfd_dropin_download()
-   host_section = .true.
-   call real_show_flow_msg(wall_time)
-   host_section = .false.
+   if (.not. diag_on_device) then
+      call ocfd_dropin_download()
+      host_section = .true.
+      call real_show_flow_msg(wall_time)
+      host_section = .false.
+      return
+   end if
+   ! E, K, S averaged over the block (device reduction), reported as
+   ! show_flow_msg reports them (OCFD_Time_Adv.f90:91-101)
+   t0 = wall_time
+   wall_time = MPI_wtime()
+   call ocfd_check(ocfd_show_flow_msg(hdev, e), 'ocfd_show_flow_msg')
+   if (my_id .eq. 0) then
+      print *, 'Istep=', Istep, 'tt=', tt
+      print *, 'CPU time per', Para%Istep_show, 'step is:    ', wall_time - t0
+      write (*, "(3A25)") "Averaged Total-energy E", "Kinetic-energy K", "Entropy S"
+      write (*, "(3E25.15)") e(1), e(2), e(3)
+      open (66, file='opencfd.log', position='append')
+      write (66, *) 'Istep=', Istep, 'tt=', tt, 'CPU time is', wall_time - t0
+      write (66, "(3A25)") "Averaged Total-energy E", "Kinetic-energy K", "Entropy S"
+      write (66, "(3E25.15)") e(1), e(2), e(3)
+      close (66)
+   end if
 end subroutine wrap_show_flow_msg
+
+! ana_box's report from the device statistics (OCFD_ANA.f90:295-310):
+! s(1:20) in the column order of statistics.dat
+subroutine ocfd_dropin_ana_box()
+   use flow_data, only: tt, my_id  ! (its constant OCFD_ANA_BOX would hide the entry point)
+   use ocfd_hip
+   use ocfd_dropin_state
+   implicit none
+   real(c_double) :: s(20)
+   call ocfd_check(ocfd_ana_box(hdev, s), 'ocfd_ana_box')
+   s(1) = tt
+   if (my_id .eq. 0) then
+      write (*, '(4(1X,A13))') "tt", "Re_lamda", "Re_lamdax", "Amt"
+      write (*, '(4(1X,E13.6E2))') s(1:4)
+      write (*, '(4(1X,A13))') "vv_rms", "u_rms", "v_rms", "w_rms"
+      write (*, '(4(1X,E13.6E2))') s(5:8)
+      write (*, '(4(1X,A13))') "u_skewness", "u_flatness", "ux_skewness", "ux_flatness"
+      write (*, '(4(1X,E13.6E2))') s(9:12)
+      write (*, '(4(1X,A13))') "E_kinetic", "Enstrophy", "d_rms", "T_aver"
+      write (*, '(4(1X,E13.6E2))') s(13:16)
+      write (*, '(4(1X,A13))') "Amu_aver", "d_aver", "Ux_rms", "Alamdax"
+      write (*, '(4(1X,E13.6E2))') s(17:20)
+      open (33, file='statistics.dat', position='append')
+      write (33, '(20(E20.13E2))') s
+      close (33)
+   end if
+end subroutine ocfd_dropin_ana_box
 
 subroutine wrap_ocfd_analysis() bind(C, name="__wrap_ocfd_analysis_")
    use flow_data
    use ocfd_dropin_state
    implicit none
    integer :: m, kstep
-   logical :: due
+   logical :: due, box_only
    interface
       subroutine real_ocfd_analysis() bind(C, name="__real_ocfd_analysis_")
       end subroutine
    end interface
    ! the schedule test of OCFD_analysis (OCFD_ANA.f90:9-14)
    due = .false.
+   box_only = .true.
    do m = 1, Para%ANA_Number
       kstep = nint(Para%ANA_Para(2, m))
-      if (mod(Istep, kstep) .eq. 0) due = .true.
+      if (mod(Istep, kstep) .eq. 0) then
+         due = .true.
+         if (nint(Para%ANA_Para(1, m)) /= OCFD_ANA_BOX) box_only = .false.
+      end if
    end do
-   if (due) call ocfd_dropin_download()
+   if (.not. due) return
+   if (diag_on_device .and. box_only) then
+      ! the due analyses, in their order, all ana_box
+      do m = 1, Para%ANA_Number
+         kstep = nint(Para%ANA_Para(2, m))
+         if (mod(Istep, kstep) .eq. 0) call ocfd_dropin_ana_box()
+      end do
+      return
+   end if
+   call ocfd_dropin_download()
    host_section = .true.
    call real_ocfd_analysis()
    host_section = .false.

@@ -141,6 +141,11 @@ module ocfd_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: h
       end function
+      integer(c_int) function ocfd_set_stage_fusion(h, on) bind(C, name="ocfd_set_stage_fusion")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: on
+      end function
       integer(c_int) function ocfd_filtering(h) bind(C, name="ocfd_filtering")
          import :: c_int, c_ptr
          type(c_ptr), value :: h

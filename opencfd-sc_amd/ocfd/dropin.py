@@ -1,0 +1,52 @@
+"""The reference solver with its hot path served by this library: the
+executable opencfd-sc_amd/fortran/Makefile links from the reference's own
+objects (NS_solver, OCFD_NS_Solver.f90:5-90, as the driver) and the
+iso_c_binding stubs of fortran/ocfd_dropin.f90.
+
+run() writes a case's input files (opencfd2.in, opencfd.dat, and the grid
+file where the case has one), runs the executable on one MPI rank and reads
+the reference's own timer: show_flow_msg prints the wall time of every
+Istep_show steps (OCFD_Time_Adv.f90:92), NS_solver the time of its whole
+loop (OCFD_NS_Solver.f90:79-80).
+"""
+import os
+import re
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DROPIN_EXE = os.path.join(HERE, "fortran", "_build", "opencfd_hip")
+
+
+class DropinError(RuntimeError):
+    pass
+
+
+def mpiexec():
+    for p in ("/opt/conda/bin/mpiexec", shutil.which("mpiexec") or ""):
+        if p and os.path.exists(p):
+            return p
+    return None
+
+
+def available():
+    return os.path.exists(DROPIN_EXE) and mpiexec() is not None
+
+
+def run(case, workdir, nsteps, show_every=None, env=None, timeout=900, exe=None):
+    """Run nsteps steps of `case` through the drop-in; returns
+    (stdout, intervals, total): intervals are the seconds of each block of
+    show_every steps (show_flow_msg's timer), total the loop's seconds."""
+    if not available():
+        raise DropinError("drop-in executable or mpiexec missing (make -C opencfd-sc_amd/fortran)")
+    case.cfg.istep_show = int(show_every or nsteps)
+    case.write_reference_inputs(workdir, nsteps=nsteps)
+    r = subprocess.run([mpiexec(), "-n", "1", exe or DROPIN_EXE], cwd=workdir, capture_output=True, text=True,
+                       timeout=timeout, env=dict(os.environ, **(env or {})))
+    out = r.stdout + r.stderr
+    if r.returncode != 0 or "STOP" in out or "libocfd_hip: block on GPU" not in out:
+        raise DropinError("drop-in run failed (rc %d):\n%s" % (r.returncode, out[-3000:]))
+    intervals = [float(m.group(1)) for m in re.finditer(r"CPU time per\s+\d+\s+step is:\s*(\S+)", out)]
+    m = re.search(r"total wtime cost:\s*(\S+)", out)
+    total = float(m.group(1)) if m else None
+    return out, intervals, total

@@ -172,6 +172,11 @@ class BlockBase:
     def bc(self):
         return self._call("bc")
 
+    def set_stage_fusion(self, on=True):
+        """ocfd_set_stage_fusion: the per-routine calls of one stage, issued
+        in NS_solver's order, run the whole-step stage kernels."""
+        return self._call("set_stage_fusion", ctypes.c_int32(1 if on else 0))
+
     def set_bc_boundarylayer(self, aoa=0.0, inlet=None, upper=None, nonref=None, pertb=None, wall=None):
         """ocfd_set_bc_boundarylayer: the data of the boundary-layer /
         swept-corner conditions -- inlet (5, nz, ny) and upper (5, nz, nx)

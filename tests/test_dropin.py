@@ -1,8 +1,13 @@
 """The reference solver with its hot-path routines served by the GPU library
-(opencfd-sc_amd/fortran: the reference's own objects, NS_solver unchanged,
-routine calls redirected through the iso_c_binding layer) against the
-unmodified reference executable on the same input files: every flow-field
-file, the ana_box statistics and the show_flow_msg sums must be identical."""
+(opencfd-sc_amd/fortran: the reference's own objects, NS_solver unchanged
+but for its host fn=f loop, routine calls redirected through the
+iso_c_binding layer) against the unmodified reference executable on the same
+input files: every flow-field file, the ana_box statistics and the
+show_flow_msg sums must be identical.  The stage routines run in stage
+fusion (the default; OCFD_HIP_STAGE_FUSION=0 is run too).  show_flow_msg and
+ana_box run on the host after a download (OCFD_HIP_DIAG=host) where their
+printouts are compared byte for byte; from the device sums (the default)
+they are compared as numbers."""
 import glob
 import os
 import re
@@ -38,6 +43,9 @@ def _case(name):
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from make_golden import golden_cases
     return golden_cases()[name]
+
+
+HOST_DIAG = {"OCFD_HIP_DIAG": "host"}
 
 
 def _run(exe, workdir, case, nsteps, env=None):
@@ -78,7 +86,7 @@ def test_dropin_matches_reference_files(tmp_path, name, nsteps):
     case = _case(name)
     print("drop-in case", name, case.cfg.shape, flush=True)
     out_ref = _run(pyoracle.REF_EXE, str(tmp_path / "ref"), case, nsteps)
-    out_gpu = _run(DROPIN, str(tmp_path / "gpu"), case, nsteps)
+    out_gpu = _run(DROPIN, str(tmp_path / "gpu"), case, nsteps, env=HOST_DIAG)
     assert "libocfd_hip: block on GPU" in out_gpu
     files = sorted(os.path.basename(p) for p in glob.glob(str(tmp_path / "ref" / "OCFD[0-9]*.dat")))
     assert len(files) == nsteps  # one per step; the final save rewrites the last
@@ -130,11 +138,65 @@ def test_dropin_user_bc(tmp_path, name, hook, mode):
     print("drop-in user BC", name, hook, mode, case.cfg.shape, flush=True)
     nsteps = 3
     out_ref = _run(ref, str(tmp_path / "ref"), case, nsteps)
-    out_gpu = _run(exe, str(tmp_path / "gpu"), case, nsteps, env={"OCFD_HIP_USER_BC": mode})
+    out_gpu = _run(exe, str(tmp_path / "gpu"), case, nsteps, env=dict(HOST_DIAG, OCFD_HIP_USER_BC=mode))
     want = "device wall hook" if mode == "device" else "host routine ocfd_bc_User"
     assert want in out_gpu
     _compare_files(tmp_path, "ref", "gpu", nsteps)
     assert _sums(out_ref) == _sums(out_gpu)
+
+
+def _stats(path):
+    import numpy as np
+    import pyoracle
+    return np.asarray(pyoracle.read_statistics(path))
+
+
+def _sum_values(stdout):
+    """E, K, S of every show_flow_msg printout, as numbers."""
+    return [[float(x) for x in l.split()] for l in _sums(stdout) if "percent" not in l]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,nsteps", [("hyb_c_llf", 3), ("flat_plate", 4)])
+def test_dropin_device_diagnostics_and_per_routine_kernels(tmp_path, name, nsteps):
+    """Defaults (stage fusion, show_flow_msg / ana_box from the device sums)
+    and the per-routine kernel sequence (OCFD_HIP_STAGE_FUSION=0) against the
+    reference: flow files byte for byte in both; the device diagnostics
+    within 1e-12 of the reference's printed E, K, S and 1e-10 of its
+    statistics.dat (both sum in another order; 1e-6 for the odd moments)."""
+    import numpy as np
+    import pyoracle
+    if not gpu_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(DROPIN) or not os.path.exists(pyoracle.REF_EXE):
+        pytest.skip("drop-in or reference executable not built")
+    case = _case(name)
+    print("drop-in device diagnostics", name, case.cfg.shape, flush=True)
+    out_ref = _run(pyoracle.REF_EXE, str(tmp_path / "ref"), case, nsteps)
+    out_dev = _run(DROPIN, str(tmp_path / "gpu"), case, nsteps)
+    _compare_files_only(tmp_path, "ref", "gpu", nsteps)
+    e_ref, e_dev = _sum_values(out_ref), _sum_values(out_dev)
+    assert len(e_ref) == len(e_dev) >= nsteps
+    np.testing.assert_allclose(e_dev, e_ref, rtol=1e-12, atol=0)
+    s_ref, s_dev = _stats(str(tmp_path / "ref" / "statistics.dat")), _stats(str(tmp_path / "gpu" / "statistics.dat"))
+    assert s_ref.shape == s_dev.shape
+    # odd moments (u and ux skewness, columns 9 and 11) are sums of terms of
+    # both signs: their relative rounding is not bounded by the sum's size
+    rtol = np.full(20, 1e-10)
+    rtol[[8, 10]] = 1e-6
+    rel = np.abs(s_dev - s_ref) / np.abs(s_ref)
+    assert np.all(rel <= rtol), rel.max(axis=0)
+    out_pr = _run(DROPIN, str(tmp_path / "routines"), case, nsteps,
+                  env=dict(HOST_DIAG, OCFD_HIP_STAGE_FUSION="0"))
+    _compare_files(tmp_path, "ref", "routines", nsteps)
+    assert _sums(out_ref) == _sums(out_pr)
+
+
+def _compare_files_only(tmp_path, ref, gpu, nsteps):
+    files = sorted(os.path.basename(p) for p in glob.glob(str(tmp_path / ref / "OCFD[0-9]*.dat")))
+    assert len(files) == nsteps
+    for fn in files:
+        assert open(tmp_path / ref / fn, "rb").read() == open(tmp_path / gpu / fn, "rb").read(), fn
 
 
 def test_fortran_binding_config_layout(tmp_path):

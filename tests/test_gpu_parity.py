@@ -72,6 +72,39 @@ def test_golden_per_routine_bitwise(name):
 
 
 @pytest.mark.parametrize("name", NAMES)
+def test_golden_stage_fusion_bitwise(name):
+    """The per-routine calls in the driver's order with stage fusion on
+    (ocfd_set_stage_fusion: du_viscous forms the fluxes, time_adv_rk3 runs
+    the whole-step stage update, comput_duvwT repairs and counts) against
+    the reference, over three steps.  Within a stage the arrays are not
+    comparable: the fused update refreshes f on the boundary planes at
+    time_adv_rk3, the reference in OCFD_bc."""
+    case, ref = load_golden(name)
+    with gpu_block(case) as b:
+        b.set_stage_fusion(True)
+        b.save_fn()
+        b.exchange_primitives()
+        if case.cfg.scheme_invis.upper() == "HYBRID":
+            b.comput_rhybrid()
+        if case.cfg.if_scheme_character:
+            b.du_inviscous_character()
+        else:
+            b.du_inviscous()
+        if case.cfg.if_viscous:
+            b.du_viscous()
+        b.time_adv_rk3(1)
+        b.comput_duvwt()
+        b.bc()
+        for krk in (2, 3):
+            b.stage(krk)
+        b.advance_clock()
+        b.filtering()
+        assert_same(b.download_f(), ref["n1_f"], "f after one step")
+        b.step_by_routines(2)
+        assert_same(b.download_f(), ref["nN_f"], "f after three steps")
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_golden_whole_steps_bitwise(name):
     """ocfd_step (fused stage kernels) over three steps against the reference."""
     case, ref = load_golden(name)
