@@ -302,6 +302,17 @@ int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape
 /* Number of scheme groups, their tile shapes (3 per group), the total cell
  * count and the marching segment length (any pointer may be NULL). */
 int ocfd_batch_info(ocfd_batch *b, int32_t *ngroups, int32_t *shapes, int64_t *cells, int32_t *kseg);
+/* How ocfd_batch_step launches a step.  mode 0 (the default): one launch
+ * per kernel phase over the tiles of all boxes.  mode 1: one persistent
+ * launch per time step: a fixed grid of workgroups takes the (box, phase,
+ * tile) items of the step's three stages from a device work queue in phase
+ * order, each item waiting only for its own box's previous phase (per-box
+ * completion counters), so boxes overlap their phases and no barrier over
+ * all boxes remains.  Mode 1 runs x tiles of 64 x 4 and y / z marches of
+ * 32-point segments (not the autotuned shapes) and needs non-characteristic
+ * schemes and CD6 in every box.  Mode 1 is currently switched off (the call
+ * fails): its kernel did not finish on the GPU. */
+int ocfd_batch_set_launch_mode(ocfd_batch *b, int32_t mode);
 
 /* ---- instrumentation -------------------------------------------------- */
 /* Enable per-kernel HIP-event timing on the handle's stream. */

@@ -1378,6 +1378,19 @@ struct ocfd_batch {
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step starting from table [q]
   int64_t cells = 0;
+  // one persistent launch per step (launch mode 1): the queue of every
+  // stage's phases (k_batch_queue), rebuilt when the segment length changes
+  int mode = 0;
+  bool queue_ok = false;      // every group runs in the queue kernel
+  std::string queue_why;      // why not
+  Launch qrep;                // repair items: 1 + the halo-copy tiles of each box
+  int qkq = -1;               // segment length the plan was built for
+  std::vector<QSeg> qseg;
+  int qitems = 0, qgrid = 0;
+  QSeg *dqseg = nullptr;
+  int *dqneed = nullptr;      // [box][Q_NKIND]
+  int *dqstate = nullptr;     // [0] ticket counter, [1] error flag, then the done counters
+  hipGraphExec_t qexec[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1409,6 +1422,10 @@ void batch_invalidate(ocfd_batch *b) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
+  for (hipGraphExec_t &g : b->qexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 void batch_free(ocfd_batch *b) {
@@ -1418,6 +1435,9 @@ void batch_free(ocfd_batch *b) {
     if (t) (void)hipFree(t);
   if (b->dtab) (void)hipFree(b->dtab);
   if (b->neg) (void)hipFree(b->neg);
+  if (b->dqseg) (void)hipFree(b->dqseg);
+  if (b->dqneed) (void)hipFree(b->dqneed);
+  if (b->dqstate) (void)hipFree(b->dqstate);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
 }
@@ -1464,19 +1484,125 @@ int batch_step_body(ocfd_batch *b, int q0) {
   return 0;
 }
 
+// The queue of one step (launch mode 1): per stage, the halo fill, the
+// sweeps of every scheme group along x, y, z, the viscous fluxes, the stage
+// updates and the repairs, each a segment over the tiles of its boxes (the
+// tables of the per-phase launches), with the items each box has of every
+// kind.  Built for the current segment length of the k-marching kernels.
+int batch_queue_plan(ocfd_batch *b) {
+  if (b->qkq == b->kq) return 0;
+  const int nbox = (int)b->box.size();
+  const int kseg = BATCH_KSEGS[b->kq];
+  std::vector<QSeg> segs;
+  int64_t first = 0;
+  auto add = [&](int kind, int stage, const ocfd_batch::Launch &l, int sch, int visc) {
+    QSeg q = {};
+    q.kind = kind;
+    q.stage = stage;
+    q.sch = sch;
+    q.visc = visc;
+    q.ids = l.ids;
+    q.off = l.off;
+    q.m = l.m;
+    q.first = (int)first;
+    q.ntiles = (int)l.ntiles;
+    first += l.ntiles;
+    segs.push_back(q);
+  };
+  for (int s = 1; s <= 3; s++) {
+    add(Q_HALO, s, b->halo, 0, 0);
+    for (int a = 0; a < 3; a++)
+      for (const auto &g : b->sweep) add(Q_X + a, s, g.l[a][queue_shape(a)], g.sch, 0);
+    for (const auto &v : b->vis)
+      if (v.visc) add(Q_VFLUX, s, v.l[b->kq], 0, 1);
+    for (const auto &v : b->vis) add(Q_UPDATE, s, v.l[b->kq], 0, v.visc ? 1 : 0);
+    add(Q_REPAIR, s, b->qrep, 0, 0);
+  }
+  if (first >= INT32_MAX / 2) return fail("ocfd_batch: too many queue items for one step");
+  std::vector<int> need((size_t)nbox * Q_NKIND, 0);
+  for (int i = 0; i < nbox; i++) {
+    const Geom &g = b->box[i]->B.g;
+    int *n = &need[(size_t)i * Q_NKIND];
+    n[Q_HALO] = (int)halo_tiles(g);
+    for (int a = 0; a < 3; a++) n[Q_X + a] = (int)sweep_tiles(g, a, queue_shape(a));
+    n[Q_VFLUX] = b->box[i]->cfg.if_viscous == 1 ? (int)march_tiles(g, kseg) : 0;
+    n[Q_UPDATE] = (int)march_tiles(g, kseg);
+    n[Q_REPAIR] = 1 + (int)halo_copy_tiles(g);
+  }
+  // every box's items in each segment are the count its counters wait for
+  std::vector<int> seen((size_t)nbox * 3 * Q_NKIND, 0);
+  for (const QSeg &q : segs)
+    for (int j = 0; j < q.m; j++) {
+      const int box = b->tab[q.ids + j], cnt = b->tab[q.off + j + 1] - b->tab[q.off + j];
+      if (box < 0 || box >= nbox || cnt != need[(size_t)box * Q_NKIND + q.kind])
+        return fail("ocfd_batch: queue plan inconsistent (segment kind " + std::to_string(q.kind) + ")");
+      seen[((size_t)box * 3 + q.stage - 1) * Q_NKIND + q.kind] += 1;
+    }
+  for (int i = 0; i < nbox; i++)
+    for (int s = 0; s < 3; s++)
+      for (int k = 0; k < Q_NKIND; k++)
+        if (seen[((size_t)i * 3 + s) * Q_NKIND + k] != (need[(size_t)i * Q_NKIND + k] > 0 ? 1 : 0))
+          return fail("ocfd_batch: queue plan misses box " + std::to_string(i));
+  HIPCHK(hipSetDevice(b->device));
+  if (!b->dqseg) {
+    HIPCHK(hipMalloc(&b->dqseg, 64 * sizeof(QSeg)));
+    HIPCHK(hipMalloc(&b->dqneed, need.size() * sizeof(int)));
+    HIPCHK(hipMalloc(&b->dqstate, (2 + (size_t)nbox * 3 * Q_NKIND) * sizeof(int)));
+  }
+  if (segs.size() > 64) return fail("ocfd_batch: too many queue segments");
+  HIPCHK(hipMemcpy(b->dqseg, segs.data(), segs.size() * sizeof(QSeg), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->dqneed, need.data(), need.size() * sizeof(int), hipMemcpyHostToDevice));
+  // captured queue steps hold the old item counts
+  for (hipGraphExec_t &gx : b->qexec) {
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+  }
+  b->qseg = segs;
+  b->qitems = (int)first;
+  b->qgrid = batch_queue_grid(b->prec, b->device);
+  if (b->qgrid <= 0) return fail("ocfd_batch: cannot size the queue kernel's grid");
+  b->qkq = b->kq;
+  return 0;
+}
+
+// one step of every box as one persistent launch (after clearing the
+// ticket and completion counters with a fill kernel)
+int batch_queue_body(ocfd_batch *b, int q0) {
+  const int nbox = (int)b->box.size();
+  const int ncnt = nbox * 3 * Q_NKIND;
+  hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ncnt + 127) / 128)), dim3(128), 0, b->st, b->dqstate + 2, ncnt);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, b->st, b->dqstate, 1);
+  QPlan P = {};
+  P.blk0 = b->dblk[0];
+  P.blk1 = b->dblk[1];
+  P.q0 = q0;
+  P.tab = b->dtab;
+  P.seg = b->dqseg;
+  P.nseg = (int)b->qseg.size();
+  P.nitems = b->qitems;
+  P.head = b->dqstate;
+  P.err = b->dqstate + 1;
+  P.done = b->dqstate + 2;
+  P.need = b->dqneed;
+  P.neg = b->neg;
+  const hipError_t e = launch_batch_queue(P, b->prec, b->qgrid, b->st);
+  return e == hipSuccess ? 0 : fail(std::string("batched queue step: ") + hipGetErrorString(e));
+}
+
 int batch_capture(ocfd_batch *b, int q0) {
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  const int rc = batch_step_body(b, q0);
+  const int rc = b->mode == 1 ? batch_queue_body(b, q0) : batch_step_body(b, q0);
   const hipError_t e = hipStreamEndCapture(b->st, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc ? -1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
-  const hipError_t ei = hipGraphInstantiate(&b->gexec[q0], graph, nullptr, nullptr, 0);
+  hipGraphExec_t *slot = b->mode == 1 ? &b->qexec[q0] : &b->gexec[q0];
+  const hipError_t ei = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) {
-    b->gexec[q0] = nullptr;
+    *slot = nullptr;
     return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
   return 0;
@@ -1562,6 +1688,14 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   for (int i = 0; i < nbox; i++) all[i] = i;
   b->halo = add_launch(b, all, [](const Geom &g) { return halo_tiles(g); });
   b->hcopy = add_launch(b, all, [](const Geom &g) { return halo_copy_tiles(g); });
+  b->qrep = add_launch(b, all, [](const Geom &g) { return 1 + halo_copy_tiles(g); });
+  // the queue kernel runs the non-characteristic sweeps and CD6
+  b->queue_ok = true;
+  for (int i = 0; i < nbox && b->queue_ok; i++) {
+    if (cfgs[i].if_scheme_character != 0) b->queue_why = "box " + std::to_string(i) + " is characteristic";
+    else if (cfgs[i].if_viscous == 1 && cfgs[i].scheme_vis == SCH_CD8) b->queue_why = "box " + std::to_string(i) + " uses CD8";
+    b->queue_ok = b->queue_why.empty();
+  }
   for (auto &g : b->sweep)
     for (int a = 0; a < 3; a++)
       for (int sh = 0; sh < NSWEEP_SHAPES; sh++)
@@ -1620,14 +1754,20 @@ int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
     HIPCHK(hipStreamSynchronize(b->st));
     return 0;
   };
+  const bool queue = b->mode == 1;
+  if (queue) {
+    if (batch_queue_plan(b)) return -1;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(128), 0, b->st, b->dqstate + 1, 1);  // error flag
+  }
   int n = 0, rc = 0;
   for (; n < nsteps && !rc; n++) {
     const int q0 = n & 1;
+    hipGraphExec_t *slot = queue ? &b->qexec[q0] : &b->gexec[q0];
     if (b->use_graph) {
-      if (!b->gexec[q0]) rc = batch_capture(b, q0);
-      if (!rc && hipGraphLaunch(b->gexec[q0], b->st) != hipSuccess) rc = fail("hipGraphLaunch failed");
+      if (!*slot) rc = batch_capture(b, q0);
+      if (!rc && hipGraphLaunch(*slot, b->st) != hipSuccess) rc = fail("hipGraphLaunch failed");
     } else {
-      rc = batch_step_body(b, q0);
+      rc = queue ? batch_queue_body(b, q0) : batch_step_body(b, q0);
     }
     // stop early once every box has stopped at too many T <= 0 cells
     if (!rc && (n + 1) % 8 == 0 && n + 1 < nsteps) {
@@ -1642,6 +1782,11 @@ int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
   }
   if (rc) return -1;
   if (read_counters()) return -1;
+  if (queue) {
+    int qerr = 0;
+    HIPCHK(hipMemcpy(&qerr, b->dqstate + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (qerr) return fail("batched queue step: a wait for a box's previous phase ran out; results are invalid");
+  }
   // the boxes' temperature buffers swapped once per stage
   for (ocfd_handle *h : b->box)
     if ((3 * n) & 1) swap_T(h);
@@ -1673,6 +1818,18 @@ int ocfd_batch_set_sweep_tiles(ocfd_batch *b, int32_t group, const int32_t shape
     if (shape[a] < 0 || shape[a] >= NSWEEP_SHAPES) return fail("sweep tile shape must be 0 .. 4");
   for (int a = 0; a < 3; a++) b->sweep[group].shape[a] = shape[a];
   batch_invalidate(b);
+  return 0;
+}
+
+int ocfd_batch_set_launch_mode(ocfd_batch *b, int32_t mode) {
+  if (mode != 0 && mode != 1) return fail("launch mode must be 0 (per phase) or 1 (one persistent launch per step)");
+  if (mode == 1 && !b->queue_ok) return fail("one persistent launch per step: " + b->queue_why);
+  // The queue kernel (k_batch_queue) did not finish on the GPU in its first
+  // runs, before and after its dependency polls were made relaxed; the
+  // cause is not found, so the mode stays off and every batch runs the
+  // per-phase launches.
+  if (mode == 1) return fail("one persistent launch per step is switched off: its kernel did not finish on the GPU");
+  b->mode = mode;
   return 0;
 }
 

@@ -15,9 +15,9 @@
 // Progress: a wait is only ever for items with smaller tickets, which were
 // taken by running workgroups or have finished, so the item with the
 // smallest unfinished ticket always has its inputs and completes.  Every
-// wait is bounded (QUEUE_SPIN_MAX polls); one that runs out sets the error
-// flag, which the host reports, and the item runs anyway, so the kernel
-// always ends.
+// wait is bounded (QUEUE_SPIN_MAX polls, about a second); one that runs out
+// sets the error flag, which the host reports, after which no item waits,
+// so the kernel always ends soon.
 //
 // Each item calls the same tile function, with the same arguments, as its
 // per-phase launch, so results are bit-identical to those launches.
@@ -30,7 +30,7 @@ namespace ocfd {
 
 namespace {
 
-constexpr int QUEUE_SPIN_MAX = 1 << 21;
+constexpr int QUEUE_SPIN_MAX = 1 << 19;
 
 // waves per SIMD the queue kernel is compiled for: those of its heaviest
 // phases (the stage update and the marches) in their own launches
@@ -103,18 +103,23 @@ void k_batch_queue(const QPlan P) {
         const int ds = dk == Q_NKIND ? stage - 1 : stage, dkind = dk == Q_NKIND ? Q_REPAIR : dk;
         const int need = P.need[box * Q_NKIND + dkind];
         const int *c = q_counter(P, box, ds, dkind);
+        // relaxed polls (an acquire on every poll would invalidate the L2
+        // each time, for every waiting workgroup), one acquire fence after
         int spins = 0;
-        while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          __builtin_amdgcn_s_sleep(2);
-          if (++spins > QUEUE_SPIN_MAX) {
-            __hip_atomic_store(P.head + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          __builtin_amdgcn_s_sleep(8);
+          // after one wait has run out, none waits any more: the kernel
+          // then drains at once
+          if ((++spins & 63) == 0 && __hip_atomic_load(P.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+          if (spins > QUEUE_SPIN_MAX) {
+            __hip_atomic_store(P.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
     switch (kind) {
       case Q_HALO: {
@@ -152,8 +157,10 @@ void k_batch_queue(const QPlan P) {
       }
     }
 
-    // publish the item: every thread's stores, then the counter
-    __threadfence();
+    // publish the item: every wave's stores (a release per wave, so that
+    // the stores of all four are complete and written back), then the
+    // counter
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_fetch_add(q_counter(P, box, stage, kind), 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

@@ -277,7 +277,8 @@ struct QPlan {
   const int *tab;
   const QSeg *seg;
   int nseg, nitems;
-  int *head;        // [0] next ticket, [1] set when a wait ran out
+  int *head;        // next ticket
+  int *err;         // set when a wait ran out (not cleared between steps)
   int *done;        // [box][stage][Q_NKIND] finished items
   const int *need;  // [box][Q_NKIND] items of each kind in one stage
   int *neg;         // T <= 0 counters, NEG_STRIDE per box

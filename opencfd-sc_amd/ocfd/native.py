@@ -421,6 +421,11 @@ class Batch:
         a = (ctypes.c_int32 * 3)(*[int(x) for x in shapes])
         return self._call("set_sweep_tiles", ctypes.c_int32(group), a)
 
+    def set_launch_mode(self, mode):
+        """0: one launch per kernel phase; 1: one persistent launch per step
+        (device work queue of (box, phase, tile) items)."""
+        return self._call("set_launch_mode", ctypes.c_int32(mode))
+
     def info(self):
         """(number of scheme groups, their tile shapes, total cells, marching
         segment length)"""

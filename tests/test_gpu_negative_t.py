@@ -88,7 +88,10 @@ def test_whole_steps_match_reference(graph, monkeypatch):
         assert b.get_time()[0] == 3
 
 
-def test_batch_box_matches_reference():
+@pytest.mark.parametrize("mode", [0, pytest.param(1, marks=pytest.mark.skip(
+    reason="the persistent queue launch is switched off: its kernel did not finish on the GPU"))],
+    ids=["per_phase", "queue"])
+def test_batch_box_matches_reference(mode):
     """The fixture's case as a box of a batch beside an unrelated box.  A
     batched box takes uniform metrics, which differ from the box generator's
     field metrics in the last bit on this block, so the expected values are
@@ -104,12 +107,14 @@ def test_batch_box_matches_reference():
     m2 = ob.step(2)
     want3 = (ob.download_f(), ob.download_primitives()[4])
     assert m1 >= 6 and m2 >= 6
-    other = cases.random_case(shape=(16, 20, 24), seed=71, scheme="WENO7", split=2, char=1)
+    # (the queue launch takes non-characteristic boxes only)
+    other = cases.random_case(shape=(16, 20, 24), seed=71, scheme="WENO7", split=2, char=0 if mode else 1)
     other.uniform_metrics = cases.uniform_box_metrics(other.cfg.nx, other.cfg.ny, other.cfg.nz)
     other.cfg.metric_mode = 1
     with native.Batch([case.cfg, other.cfg]) as bt:
         native.setup_block(bt.box(0), case, use_uniform=True)
         native.setup_block(bt.box(1), other, use_uniform=True)
+        bt.set_launch_mode(mode)
         n1 = bt.step(1)
         assert_same(bt.box(0).download_f(), want1[0], "batch box f after one step")
         assert_same(bt.box(0).download_primitives()[4], want1[1], "batch box T after one step")
