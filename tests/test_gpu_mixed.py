@@ -1,9 +1,10 @@
 """Mixed-precision form (OCFD_PREC_MIXED) against the FP64 path.
 
 The mixed form forms the flux splitting, the reconstructions of the sweeps and
-the viscous derivatives / stresses / fluxes in FP32; primitives, f, fn, du, the
-RK3 stage combination and the metrics stay FP64, and so do the faces the
-hybrid sensor flags (characteristic projection and WENO).  Its results are
+the viscous derivatives / stresses / fluxes in FP32, accumulates du in FP32
+and stores the primitives as FP32; f, fn, the RK3 stage combination and the
+metrics stay FP64, and so do the characteristic basis and projections of the
+faces the hybrid sensor flags.  Its results are
 compared with the FP64 path, which the other GPU tests pin bit for bit to the
 compiled reference and the oracle, and with the oracle directly.
 

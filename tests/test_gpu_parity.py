@@ -11,8 +11,11 @@ Tolerances (FP64 path, per field):
     (relative L2 <= 1e-14); Rhybrid is only compared with the thresholds, so
     everything downstream stays bitwise unless a value sits within an ulp of
     a threshold;
-  100-step TGV and isotropic runs against the oracle: relative L2 <= 1e-12
-    per field, kinetic-energy / enstrophy histories <= 1e-10 relative.
+  the 100-step TGV run against the oracle: relative L2 <= 1e-12 per field,
+    kinetic-energy / enstrophy histories <= 1e-10 relative; isotropic
+    periodic boxes with uniform metrics against the oracle over a few
+    steps.  The benchmark configurations (isotropic turbulence among them)
+    run against the compiled reference in test_gpu_configs.py.
 """
 import numpy as np
 import pytest
