@@ -115,7 +115,16 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   const int *list = B.hlist[AXIS];
   double *out = B.wflux[AXIS];
   const bool sw = S.flux_split == 1;
-  for (int batch = blockIdx.x; batch < nbatch; batch += gridDim.x) {
+  // batches in one contiguous range per XCD (workgroup b runs on XCD
+  // b % NXCD), strided over that XCD's workgroups: list-neighbouring faces,
+  // whose y / z stencils share rows, then meet in the same L2
+  const int nw = (int)gridDim.x / NXCD, xcd = (int)blockIdx.x % NXCD;
+  const bool split = (int)gridDim.x % NXCD == 0 && nw > 0;
+  const int blo = split ? (int)((int64_t)nbatch * xcd / NXCD) : 0;
+  const int bhi = split ? (int)((int64_t)nbatch * (xcd + 1) / NXCD) : nbatch;
+  const int b0 = split ? blo + (int)blockIdx.x / NXCD : (int)blockIdx.x;
+  const int bstep = split ? nw : (int)gridDim.x;
+  for (int batch = b0; batch < bhi; batch += bstep) {
     const int f0 = batch * FB;
     const int nf = count - f0 < FB ? count - f0 : FB;
     // split fluxes at the stencil points (OCFD_inviscous.f90:64-70, split_*)
@@ -195,12 +204,15 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
           xp[k] = sr5[0] * double(sF[fb][k * 10]);
           xm[k] = sr5[0] * double(sF[fb][k * 10 + 5]);
         }
+        // (FP64 path: the reference's two roundings; the mixed form's FP64
+        // projection fuses them)
+        auto acc = [](double a, double b, double c) { return sizeof(RR) == 8 ? madd(a, b, c) : fma(a, b, c); };
 #pragma unroll
         for (int cmp = 1; cmp < 5; cmp++) {
 #pragma unroll
           for (int k = 0; k < 8; k++) {
-            xp[k] = madd(sr5[cmp], double(sF[fb][k * 10 + cmp]), xp[k]);
-            xm[k] = madd(sr5[cmp], double(sF[fb][k * 10 + 5 + cmp]), xm[k]);
+            xp[k] = acc(sr5[cmp], double(sF[fb][k * 10 + cmp]), xp[k]);
+            xm[k] = acc(sr5[cmp], double(sF[fb][k * 10 + 5 + cmp]), xm[k]);
           }
         }
 #pragma unroll
