@@ -46,7 +46,16 @@ def run(case, workdir, nsteps, show_every=None, env=None, timeout=900, exe=None)
     out = r.stdout + r.stderr
     if r.returncode != 0 or "STOP" in out or "libocfd_hip: block on GPU" not in out:
         raise DropinError("drop-in run failed (rc %d):\n%s" % (r.returncode, out[-3000:]))
-    intervals = [float(m.group(1)) for m in re.finditer(r"CPU time per\s+\d+\s+step is:\s*(\S+)", out)]
-    m = re.search(r"total wtime cost:\s*(\S+)", out)
-    total = float(m.group(1)) if m else None
+    intervals, total = parse_timers(out)
     return out, intervals, total
+
+
+def parse_timers(out):
+    """The reference's own timers in a run's output: the seconds of every
+    block of Istep_show steps (show_flow_msg, OCFD_Time_Adv.f90:92) and of
+    the whole loop (OCFD_NS_Solver.f90:80; None if absent)."""
+    intervals = [float(m.group(1).replace("D", "E"))
+                 for m in re.finditer(r"CPU time per\s+\d+\s+step is:\s*(\S+)", out)]
+    m = re.search(r"total wtime cost:\s*(\S+)", out)
+    total = float(m.group(1).replace("D", "E")) if m else None
+    return intervals, total

@@ -230,3 +230,17 @@ end program layout
     C = OcfdConfigC
     assert vals == [ctypes.sizeof(C), C.iperiodic.offset, C.ma.offset, C.bc_para.offset,
                     C.filter_para.offset, C.reserved_d.offset]
+
+
+def test_dropin_timer_parsing():
+    """The bench's drop-in leg reads the reference's own timers from the run's
+    output (list-directed prints of show_flow_msg and NS_solver)."""
+    from ocfd import dropin
+    out = (" Istep=          20 tt=  1.1440000000000001E-002\n"
+           " CPU time per          20 step is:      0.31456789000000001\n"
+           " Istep=          40 tt=  2.2880000000000001E-002\n"
+           " CPU time per          20 step is:      0.28571234500000000\n"
+           " total wtime cost:  0.60245000000000004\n")
+    iv, total = dropin.parse_timers(out)
+    assert iv == [0.31456789000000001, 0.285712345] and total == 0.60245000000000004
+    assert dropin.parse_timers("no timers") == ([], None)
