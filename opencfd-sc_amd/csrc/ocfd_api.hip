@@ -1825,9 +1825,9 @@ int ocfd_batch_set_launch_mode(ocfd_batch *b, int32_t mode) {
   if (mode != 0 && mode != 1) return fail("launch mode must be 0 (per phase) or 1 (one persistent launch per step)");
   if (mode == 1 && !b->queue_ok) return fail("one persistent launch per step: " + b->queue_why);
   // The queue kernel (k_batch_queue) did not finish on the GPU in its first
-  // runs, before and after its dependency polls were made relaxed; the
-  // cause is not found, so the mode stays off and every batch runs the
-  // per-phase launches.
+  // two runs: its loop, split by the compiler, repeated each workgroup's
+  // first item.  The rewritten loop has not run on the GPU yet, so the mode
+  // stays off and every batch runs the per-phase launches.
   if (mode == 1) return fail("one persistent launch per step is switched off: its kernel did not finish on the GPU");
   b->mode = mode;
   return 0;

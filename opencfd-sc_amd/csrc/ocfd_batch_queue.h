@@ -80,12 +80,17 @@ template <typename R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QUEUE_WAVES<R>, QUEUE_WAVES<R>)))
 void k_batch_queue(const QPlan P) {
   __shared__ int s_ticket;
-  for (;;) {
-    if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(P.head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int tk = s_ticket;
-    __syncthreads();
-    if (tk >= P.nitems) break;
+  // Thread 0 takes the tickets, the workgroup reads them as a scalar: the
+  // loop test and every branch on the item are uniform, and the only
+  // divergent regions (thread 0's ticket, wait and counter) close inside the
+  // loop body.  With the fetch as a divergent region at the top of the loop
+  // and the ticket in a vector register, the compiler splits the loop so
+  // that all lanes but thread 0 return to the ticket barrier without thread
+  // 0's next fetch, and the workgroup repeats its first item for ever.
+  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(P.head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  int tk = __builtin_amdgcn_readfirstlane(s_ticket);
+  while (tk < P.nitems) {
     int si = 0;
     while (si + 1 < P.nseg && P.seg[si + 1].first <= tk) si++;
     const QSeg &sg = P.seg[si];
@@ -159,11 +164,15 @@ void k_batch_queue(const QPlan P) {
 
     // publish the item: every wave's stores (a release per wave, so that
     // the stores of all four are complete and written back), then the
-    // counter
+    // counter; then the next ticket
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
       __hip_atomic_fetch_add(q_counter(P, box, stage, kind), 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_ticket = __hip_atomic_fetch_add(P.head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    tk = __builtin_amdgcn_readfirstlane(s_ticket);
   }
 }
 

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(NT) void k_hyb_list(const Block B) {
 // in the mixed-precision form, where the characteristic basis and the
 // projections onto and back from it stay FP64.
 template <int AXIS, bool CHAR, bool FAST, typename RR>
-__global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(RR) == 4 ? 4 : 1)))
+void k_weno_faces(const Block B) {
   typedef TileIndex<AXIS> TI;
   constexpr int NH = CHAR ? 5 : 10;
   __shared__ RR sF[FB][SFS];        // fp(1:5), fm(1:5) at stencil points s-3 .. s+4: [p * 10 + m]
@@ -124,11 +125,69 @@ __global__ __launch_bounds__(NT) void k_weno_faces(const Block B) {
   const int bhi = split ? (int)((int64_t)nbatch * (xcd + 1) / NXCD) : nbatch;
   const int b0 = split ? blo + (int)blockIdx.x / NXCD : (int)blockIdx.x;
   const int bstep = split ? nw : (int)gridDim.x;
+  // The FP32 form with uniform metrics loads the primitives of its next
+  // batch's stencil points right after splitting the current batch's, so the
+  // loads run during the current batch's projection and reconstruction (the
+  // FP64 form has no registers to spare for them).
+  constexpr bool PF = FAST && sizeof(RR) == 4;
+  constexpr int NPI = (8 * FB + NT - 1) / NT;  // stencil points per lane and batch
+  float pv[PF ? NPI : 1][5];
+  auto fetch = [&](int bt) {
+    const int fn0 = bt * FB;
+    const int nfn = count - fn0 < FB ? count - fn0 : FB;
+#pragma unroll
+    for (int r = 0; r < NPI; r++) {
+      const int it = tid + r * NT;
+      if (bt < bhi && it < nfn * 8) {
+        const int fb = it >> 3, p = it & 7;
+        int s, w, c;
+        TI::face_of(g, list[fn0 + fb] >> 1, s, w, c);
+        const int64_t h = TI::hidx(g, s - 3 + p, w, c);
+        pv[r][0] = B.d.template as<float>()[h];
+        pv[r][1] = B.u.template as<float>()[h];
+        pv[r][2] = B.v.template as<float>()[h];
+        pv[r][3] = B.w.template as<float>()[h];
+        pv[r][4] = B.T.template as<float>()[h];
+      }
+    }
+  };
+  if constexpr (PF) fetch(b0);
   for (int batch = b0; batch < bhi; batch += bstep) {
     const int f0 = batch * FB;
     const int nf = count - f0 < FB ? count - f0 : FB;
+    if constexpr (PF) {
+      // split fluxes at the stencil points from the prefetched primitives
+      // (the same values the loop below reads: FP32 storage, widened)
+#pragma unroll
+      for (int r = 0; r < NPI; r++) {
+        const int it = tid + r * NT;
+        if (it < nf * 8) {
+          const int fb = it >> 3, p = it & 7;
+          const double dd = pv[r][0], uu = pv[r][1], vv = pv[r][2], ww = pv[r][3], Tt = pv[r][4];
+          const double A1 = B.M.u[q1], A2 = B.M.u[q1 + 1], A3 = B.M.u[q1 + 2];
+          SplitK<RR> K;
+          split_k_gamma(RR(gamma), K);
+          split_k_metric(RR(A1), RR(A2), RR(A3), sw, K);
+          const RR ccr = sqrt(RR(Tt)) / RR(rma);
+          RR fp[5], fm[5];
+          if (sw) split_sw<RR>(K, ccr, RR(dd), RR(uu), RR(vv), RR(ww), fp, fm);
+          else split_llf<RR>(K, ccr, RR(dd), RR(uu), RR(vv), RR(ww), fp, fm);
+#pragma unroll
+          for (int m = 0; m < 5; m++) {
+            sF[fb][p * 10 + m] = fp[m];
+            sF[fb][p * 10 + 5 + m] = fm[m];
+          }
+          if (CHAR && (p == 3 || p == 4)) {
+            double *q = sQ[fb][p - 3];
+            q[0] = uu; q[1] = vv; q[2] = ww; q[3] = sqrt(Tt) / rma;
+            q[4] = A1; q[5] = A2; q[6] = A3;
+          }
+        }
+      }
+      fetch(batch + bstep);
+    }
     // split fluxes at the stencil points (OCFD_inviscous.f90:64-70, split_*)
-    for (int it = tid; it < nf * 8; it += NT) {
+    for (int it = tid; !PF && it < nf * 8; it += NT) {
       const int fb = it >> 3, p = it & 7;
       int s, w, c;
       TI::face_of(g, list[f0 + fb] >> 1, s, w, c);

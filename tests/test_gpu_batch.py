@@ -116,8 +116,9 @@ def test_batch_rejects_unsupported_boxes():
         native.Batch([c2.cfg])
 
 
-QUEUE_OFF = ("the persistent queue launch is switched off: its kernel did not finish on the GPU "
-             "(twice, the second time after its dependency polls were made relaxed; cause not found)")
+QUEUE_OFF = ("the persistent queue launch is switched off: its kernel did not finish on the GPU in two runs "
+             "(its loop, split by the compiler, repeated each workgroup's first item); the rewritten loop "
+             "has not run on the GPU yet")
 
 
 @pytest.mark.skip(reason=QUEUE_OFF)
