@@ -258,6 +258,48 @@ def profile_traffic(workload: str, precision: str, name: str):
     return None, None
 
 
+def step_traffic(workload: str, precision: str):
+    """HBM bytes per time step of every kernel of the step, from the committed
+    summary of this workload and precision (rocprofv3 FETCH_SIZE + WRITE_SIZE
+    per launch times launches, over the profiled steps, i.e. stage-update
+    launches / 3; the diagnostics kernel k_hyb_counts, the runtime's copies
+    and the calibration copy kernel left out); (bytes, file) or (None, None)."""
+    fn = PROFILE_FILE.get((workload, precision))
+    if fn is None or not os.path.exists(os.path.join(ROOT, "profiles", fn)):
+        return None, None
+    total, updates = 0.0, 0
+    for line in open(os.path.join(ROOT, "profiles", fn)):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) < 5 or not cols[0].startswith("`"):
+            continue
+        name = cols[0].strip("`")
+        try:
+            n, rd, wr = int(cols[1]), float(cols[3]), float(cols[4])
+        except ValueError:
+            continue
+        if name.startswith("__amd") or name.startswith("k_hyb_counts") or name.startswith("k_copy"):
+            continue
+        if name.startswith("k_visc_div_update") or name.startswith("k_visc_fused"):
+            updates += n
+        total += n * (rd + wr) * 1e6
+    if updates == 0:
+        return None, None
+    return total / (updates / 3.0), "profiles/" + fn
+
+
+def step_hbm(workload, prec, step_s):
+    """The whole step's measured HBM traffic (committed profile of the same
+    workload) over this run's time per step, against 8 TB/s."""
+    b, src = step_traffic(workload, "mixed" if prec == PREC_MIXED else "fp64")
+    if b is None:
+        return None
+    gbs = b / step_s / 1e9
+    return {"bytes_per_step": round(b), "achieved_GBs": round(gbs, 1), "peak_GBs": perfmodel.PEAK_HBM_GBS,
+            "frac": round(gbs / perfmodel.PEAK_HBM_GBS, 4),
+            "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of every kernel of the step in the committed summary %s "
+                      "(not collected in this run), over this run's time per step" % src}
+
+
 def kernel_roofline(name, ms, cnt, cfg, variant, mix, prec, workload):
     """The kernel `name` on its roofline: compulsory bytes (stage update:
     120 B per cell, 3 x 5 FP64 values) or algorithmic FLOPs per launch over
@@ -575,6 +617,7 @@ def main():
         name, ms, cnt = dominant(r["prof"])
         out = {"roofline": kernel_roofline(name, ms, cnt, cfg, variant, r["mix"], pr, args.workload),
                "step_roofline": step_roofline(r["rate"], cfg, variant, r["mix"], pr),
+               "step_hbm_measured": step_hbm(args.workload, pr, r["elapsed"] / args.steps),
                "kernels_ms_per_step": {k: round(v[0] / args.prof_steps, 4) for k, v in r["prof"].items()}}
         if r["mix"]:
             out["face_mix_linear_weno7_weno5"] = [round(x, 5) for x in r["mix"]]
@@ -591,6 +634,7 @@ def main():
         "dtype": "f64" if prec == PREC_FP64 else "f32+f64",
         "data": "synthetic (seeded initial field generated on the host)", "config": desc,
         "roofline": rep.pop("roofline"), "step_roofline": rep.pop("step_roofline"),
+        "step_hbm_measured": rep.pop("step_hbm_measured"),
     }
     res["config"]["precision"] = ("FP64" if prec == PREC_FP64 else
                                   "mixed: FP32 splitting, reconstruction, viscous derivatives, du and stored "
