@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Throughput of the OpenCFD-SC right-hand side + RK3 step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload iso512]
-                    [--precision fp64|mixed] [--cpu-baseline auto|reference|port|none]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload iso512|iso256|tgv64|batch96]
+                    [--precision fp64|mixed] [--cpu-baseline auto|reference|port|none] [--dropin 0|1]
 
 A step is one full RK3 time step (three right-hand-side evaluations, stage
 updates, boundary refresh, filtering when configured) of one block already
@@ -30,6 +30,14 @@ run the same steps.  Prints one JSON line:
   step_roofline  the whole step: 360 B per cell-step against 8 TB/s and the
              exact FLOPs per cell-step at the measured face mix against
              78.6 TFLOP/s
+  step_hbm_measured  the whole step's HBM bytes as rocprofv3 counted them in
+             the committed summary of this workload, over this run's time
+             per step, against 8 TB/s
+  dropin     the reference's own driver (NS_solver) with this library as its
+             hot path, on 256^3 isotropic turbulence with field metrics,
+             timed by the reference's show_flow_msg timer, beside
+             ocfd_step on the same case (--dropin 0 leaves it out; it runs
+             mpiexec and must stay out of profiler runs)
   cpu_baseline  the compiled reference solver under MPI on this host's cores
              (median of 3) and on one rank, on a field developed on the GPU
              to the same physical time (falls back to the single-threaded C
@@ -575,8 +583,8 @@ def main():
                          "read and written) this many times: the reference for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        log("bench.py measures one MI355X: every box in this setup has one GPU, so --gpus %d is not "
-            "supported (the multi-GPU halo-exchange path is design-only)" % args.gpus)
+        log("bench.py measures one MI355X: --gpus %d is not supported (one GPU runs the whole block; the "
+            "multi-GPU halo-exchange path is design-only)" % args.gpus)
         sys.exit(2)
 
     if args.workload.startswith("batch"):

@@ -19,7 +19,7 @@ cores inside the test on the same input files.  Each case runs once.
   C5  the 96-box batch in mixed precision against the same boxes stepped
       alone (bitwise), and sampled boxes against the FP64 oracle
 
-Bars (north star): FP64 relative L2 per field <= 1e-12 and kinetic-energy /
+Bars: FP64 relative L2 per field <= 1e-12 and kinetic-energy /
 enstrophy histories (ana_box every 10 steps, statistics.dat) <= 1e-10;
 mixed precision <= 1e-5 against FP64.
 """
