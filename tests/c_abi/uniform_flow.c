@@ -2,8 +2,11 @@
  * include/ocfd_hip.h: a periodic 32^3 box of uniform flow, hybrid
  * characteristic scheme with LLF splitting and CD6 viscous terms, 20 whole
  * steps, then the primitives back.  A uniform state has zero right-hand
- * side, so the state must come back unchanged bit for bit.  Exit status:
- * 0 = unchanged, 1 = the library refused (message printed), 2 = changed. */
+ * side, so the state must come back unchanged up to the rounding of the RK3
+ * stage combinations and of the primitive recovery (relative 1e-13).  Exit
+ * status: 0 = unchanged, 1 = the library refused (message printed),
+ * 2 = changed. */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -60,8 +63,13 @@ int main(void) {
     ocfd_get_time(h, &istep, &tt);
     ocfd_destroy(h);
     size_t changed = 0;
+    double worst = 0.0;
     for (int m = 0; m < 5; m++)
-        for (size_t p = 0; p < np; p++) changed += r[m][p] != q[m][p];
-    printf("steps %d, t %.6f, values changed %zu\n", istep, tt, changed);
+        for (size_t p = 0; p < np; p++) {
+            const double e = fabs(r[m][p] - q[m][p]) / fabs(q[m][p]);
+            worst = e > worst ? e : worst;
+            changed += !(e <= 1e-13);
+        }
+    printf("steps %d, t %.6f, largest relative change %.3e, values changed %zu\n", istep, tt, worst, changed);
     return changed ? 2 : 0;
 }
