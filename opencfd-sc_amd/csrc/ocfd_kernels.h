@@ -152,17 +152,23 @@ struct TileIndex {
     if (AXIS == 0) return ((int64_t)(c - 1) * g.ny + (w - 1)) * (g.nx + 1) + s;
     return ((int64_t)(c - 1) * (g.n(AXIS) + 1) + s) * g.nx + (w - 1);
   }
+  // in 32-bit unsigned arithmetic: only the hybrid scheme's face lists use
+  // it, and ocfd_create refuses hybrid blocks with 2^30 or more faces per
+  // axis (the 64-bit divisions took a tenth of the list kernel's time)
   __device__ static inline void face_of(const Geom &g, int64_t fid, int &s, int &w, int &c) {
+    const unsigned f = (unsigned)fid;
     if (AXIS == 0) {
-      s = (int)(fid % (g.nx + 1));
-      const int64_t r = fid / (g.nx + 1);
-      w = (int)(r % g.ny) + 1;
-      c = (int)(r / g.ny) + 1;
+      const unsigned nl = (unsigned)g.nx + 1u, r = f / nl;
+      s = (int)(f - r * nl);
+      const unsigned r2 = r / (unsigned)g.ny;
+      w = (int)(r - r2 * (unsigned)g.ny) + 1;
+      c = (int)r2 + 1;
     } else {
-      w = (int)(fid % g.nx) + 1;
-      const int64_t r = fid / g.nx;
-      s = (int)(r % (g.n(AXIS) + 1));
-      c = (int)(r / (g.n(AXIS) + 1)) + 1;
+      const unsigned r = f / (unsigned)g.nx;
+      w = (int)(f - r * (unsigned)g.nx) + 1;
+      const unsigned nl = (unsigned)g.n(AXIS) + 1u, r2 = r / nl;
+      s = (int)(r - r2 * nl);
+      c = (int)r2 + 1;
     }
   }
 };
