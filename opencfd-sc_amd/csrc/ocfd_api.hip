@@ -1348,9 +1348,13 @@ struct ocfd_batch {
   int prec = OCFD_PREC_FP64_CODE;
   std::vector<ocfd_handle *> box;
   // device copies of the boxes' Block descriptors: [0] as the boxes hold
-  // them when a call starts, [1] with T and T2 swapped; stage s of step n
-  // reads table (n + s) % 2 (every stage swaps the temperature buffers)
+  // them when a call starts, [1] with T and T2 swapped (and u, v, w with
+  // u2, v2, w2 for the boxes whose stages run k_visc_fused); stage s of step
+  // n reads table (n + s) % 2 (every stage swaps those buffers).  drep[q]
+  // is table q with the fused boxes' u, v, w sets swapped: the T <= 0
+  // repair after a stage sees their new u, v, w beside the old T
   Block *dblk[2] = {nullptr, nullptr};
+  Block *drep[2] = {nullptr, nullptr};
   int *dtab = nullptr;        // box lists and tile offsets of every launch
   int *neg = nullptr;         // NEG_STRIDE ints per box
   std::vector<int> tab;       // host image of dtab
@@ -1365,8 +1369,9 @@ struct ocfd_batch {
     Launch l[3][NSWEEP_SHAPES];
     int shape[3] = {0, 0, 0};
   };
-  struct ViscGroup {          // boxes sharing IF_Viscous and the CD order
+  struct ViscGroup {          // boxes sharing IF_Viscous, the CD order and the fused update
     bool visc = false, cd8 = false;
+    bool fused = false;         // k_visc_fused (the boxes' handles run it alone too)
     std::vector<int> ids;
     Launch l[NBATCH_KSEG];     // per segment length BATCH_KSEGS[q]
   };
@@ -1395,8 +1400,8 @@ struct ocfd_batch {
 
 namespace {
 
-BoxSet boxset(const ocfd_batch *b, const ocfd_batch::Launch &l, int table = 0) {
-  return BoxSet{b->dblk[table], b->dtab + l.ids, b->dtab + l.off, l.m};
+BoxSet boxset(const ocfd_batch *b, const ocfd_batch::Launch &l, int table = 0, bool repair = false) {
+  return BoxSet{repair ? b->drep[table] : b->dblk[table], b->dtab + l.ids, b->dtab + l.off, l.m};
 }
 
 // box list + tile offsets of one launch, appended to the host table
@@ -1433,6 +1438,8 @@ void batch_free(ocfd_batch *b) {
   for (ocfd_handle *h : b->box) ocfd_destroy(h);
   for (Block *t : b->dblk)
     if (t) (void)hipFree(t);
+  for (Block *t : b->drep)
+    if (t) (void)hipFree(t);
   if (b->dtab) (void)hipFree(b->dtab);
   if (b->neg) (void)hipFree(b->neg);
   if (b->dqseg) (void)hipFree(b->dqseg);
@@ -1466,18 +1473,20 @@ int batch_step_body(ocfd_batch *b, int q0) {
       for (int a = 0; a < 3; a++)
         if (batch_sweep(b, g, a, g.shape[a], q)) return -1;
     for (const auto &v : b->vis) {
-      if (!v.visc) continue;
+      if (!v.visc || v.fused) continue;
       const auto &l = v.l[b->kq];
       const hipError_t e = launch_visc_flux_batch(boxset(b, l, q), b->prec, v.cd8, (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched viscous flux: ") + hipGetErrorString(e));
     }
     for (const auto &v : b->vis) {
       const auto &l = v.l[b->kq];
-      const hipError_t e = launch_visc_div_update_batch(boxset(b, l, q), b->prec, v.visc, v.cd8, krk, b->neg,
-                                                        (int)l.ntiles, b->st);
+      const hipError_t e =
+          v.fused ? launch_visc_fused_batch(boxset(b, l, q), krk, b->neg, (int)l.ntiles, b->st)
+                  : launch_visc_div_update_batch(boxset(b, l, q), b->prec, v.visc, v.cd8, krk, b->neg,
+                                                 (int)l.ntiles, b->st);
       if (e != hipSuccess) return fail(std::string("batched stage update: ") + hipGetErrorString(e));
     }
-    const hipError_t e = launch_negt_repair_batch(boxset(b, b->hcopy, q), (int)b->box.size(), b->neg,
+    const hipError_t e = launch_negt_repair_batch(boxset(b, b->hcopy, q, true), (int)b->box.size(), b->neg,
                                                   (int)b->hcopy.ntiles, b->st);
     if (e != hipSuccess) return fail(std::string("batched T <= 0 repair: ") + hipGetErrorString(e));
   }
@@ -1612,19 +1621,31 @@ int batch_capture(ocfd_batch *b, int q0) {
 // own streams are drained first (set-up work may still run there)
 int batch_sync_tables(ocfd_batch *b) {
   HIPCHK(hipSetDevice(b->device));
-  std::vector<Block> blk[2];
+  std::vector<Block> blk[2], rep[2];
   for (size_t i = 0; i < b->box.size(); i++) {
     ocfd_handle *h = b->box[i];
     HIPCHK(hipStreamSynchronize(h->st));
     Block B = h->B;
     B.kseg = BATCH_KSEGS[b->kq];
     B.halt = b->neg + NEG_STRIDE * i + NEG_HALT;
+    auto swap_uvw_of = [&](Block &x) {
+      if (!h->fused) return;
+      std::swap(x.u, x.u2);
+      std::swap(x.v, x.v2);
+      std::swap(x.w, x.w2);
+    };
     blk[0].push_back(B);
+    swap_uvw_of(B);
+    rep[0].push_back(B);
     std::swap(B.T, B.T2);
     blk[1].push_back(B);
+    swap_uvw_of(B);
+    rep[1].push_back(B);
   }
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 2; q++) {
     HIPCHK(hipMemcpy(b->dblk[q], blk[q].data(), blk[q].size() * sizeof(Block), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->drep[q], rep[q].data(), rep[q].size() * sizeof(Block), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemcpy(b->dtab, b->tab.data(), b->tab.size() * sizeof(int), hipMemcpyHostToDevice));
   return 0;
 }
@@ -1674,12 +1695,17 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
     }
     it->ids.push_back(i);
     const bool visc = c.if_viscous == 1, cd8 = c.scheme_vis == SCH_CD8;
-    auto iv = std::find_if(b->vis.begin(), b->vis.end(),
-                           [&](const ocfd_batch::ViscGroup &g) { return g.visc == visc && g.cd8 == cd8; });
+    // a box that steps alone with k_visc_fused (mixed precision, whole 32 x 8
+    // tiles, CD6; its second u, v, w buffers exist) runs it in the batch too
+    const bool fused = b->box[i]->fused;
+    auto iv = std::find_if(b->vis.begin(), b->vis.end(), [&](const ocfd_batch::ViscGroup &g) {
+      return g.visc == visc && g.cd8 == cd8 && g.fused == fused;
+    });
     if (iv == b->vis.end()) {
       b->vis.emplace_back();
       b->vis.back().visc = visc;
       b->vis.back().cd8 = cd8;
+      b->vis.back().fused = fused;
       iv = b->vis.end() - 1;
     }
     iv->ids.push_back(i);
@@ -1694,6 +1720,7 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   for (int i = 0; i < nbox && b->queue_ok; i++) {
     if (cfgs[i].if_scheme_character != 0) b->queue_why = "box " + std::to_string(i) + " is characteristic";
     else if (cfgs[i].if_viscous == 1 && cfgs[i].scheme_vis == SCH_CD8) b->queue_why = "box " + std::to_string(i) + " uses CD8";
+    else if (b->box[i]->fused) b->queue_why = "box " + std::to_string(i) + " runs the fused viscous stage update";
     b->queue_ok = b->queue_why.empty();
   }
   for (auto &g : b->sweep)
@@ -1704,7 +1731,9 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
         });
   for (auto &v : b->vis)
     for (int q = 0; q < NBATCH_KSEG; q++)
-      v.l[q] = add_launch(b, v.ids, [q](const Geom &g) { return march_tiles(g, BATCH_KSEGS[q]); });
+      v.l[q] = add_launch(b, v.ids, [q, &v](const Geom &g) {
+        return v.fused ? fused_batch_tiles(g, BATCH_KSEGS[q]) : march_tiles(g, BATCH_KSEGS[q]);
+      });
   int64_t maxt = 0;
   for (int x : b->tab) maxt = std::max<int64_t>(maxt, x);
   if (maxt >= INT32_MAX / 2) {
@@ -1715,6 +1744,7 @@ int ocfd_batch_create(const ocfd_config *cfgs, int32_t nbox, int32_t device, ocf
   rc |= hipSetDevice(device) != hipSuccess;
   rc |= hipStreamCreate(&b->st) != hipSuccess;
   for (int q = 0; q < 2; q++) rc |= hipMalloc(&b->dblk[q], nbox * sizeof(Block)) != hipSuccess;
+  for (int q = 0; q < 2; q++) rc |= hipMalloc(&b->drep[q], nbox * sizeof(Block)) != hipSuccess;
   rc |= hipMalloc(&b->dtab, b->tab.size() * sizeof(int)) != hipSuccess;
   rc |= hipMalloc(&b->neg, (size_t)NEG_STRIDE * nbox * sizeof(int)) != hipSuccess;
   if (rc) {
@@ -1787,9 +1817,13 @@ int ocfd_batch_step(ocfd_batch *b, int32_t nsteps) {
     HIPCHK(hipMemcpy(&qerr, b->dqstate + 1, sizeof(int), hipMemcpyDeviceToHost));
     if (qerr) return fail("batched queue step: a wait for a box's previous phase ran out; results are invalid");
   }
-  // the boxes' temperature buffers swapped once per stage
+  // the boxes' temperature buffers (and the fused boxes' u, v, w sets)
+  // swapped once per stage
   for (ocfd_handle *h : b->box)
-    if ((3 * n) & 1) swap_T(h);
+    if ((3 * n) & 1) {
+      swap_T(h);
+      if (h->fused) swap_uvw(h);
+    }
   int total = 0;
   std::string err;
   for (int i = 0; i < nbox; i++) {
@@ -1875,7 +1909,9 @@ int ocfd_batch_autotune(ocfd_batch *b, int32_t reps) {
   // segment length of the k-marching kernels, timed on the viscous flux
   // kernel (it writes only Ev); the descriptors are re-sent per candidate
   bool anyvisc = false;
-  for (const auto &v : b->vis) anyvisc |= v.visc;
+  // (the fused groups' update changes the state, so only the viscous
+  // flux of the others is timed; the fused march takes at least 16 planes)
+  for (const auto &v : b->vis) anyvisc |= v.visc && !v.fused;
   if (!rc && anyvisc) {
     float best = 0.f;
     int bestq = b->kq;
@@ -1887,7 +1923,7 @@ int ocfd_batch_autotune(ocfd_batch *b, int32_t reps) {
       }
       float tot = 0.f;
       for (const auto &v : b->vis) {
-        if (!v.visc) continue;
+        if (!v.visc || v.fused) continue;
         const auto &l = v.l[q];
         hipError_t e = launch_visc_flux_batch(boxset(b, l), b->prec, v.cd8, (int)l.ntiles, b->st);
         e = e != hipSuccess ? e : hipEventRecord(e0, b->st);

@@ -254,7 +254,9 @@ hipError_t launch_visc_div_update_batch(const BoxSet &S, int prec, bool visc, bo
                                         int ntiles, hipStream_t st);
 hipError_t launch_halo_batch(const BoxSet &S, int ntiles, hipStream_t st);
 // the T <= 0 repair of every box (workgroup b repairs box b of the table) and
-// the copy of T's halo into T2, over the halo-copy tiles of S
+// the copy of T's halo into T2, over the halo-copy tiles of S (for boxes
+// with second u, v, w buffers, i.e. a fused stage, also the halos of u2, v2,
+// w2 into u, v, w)
 hipError_t launch_negt_repair_batch(const BoxSet &S, int nbox, int *neg, int ntiles, hipStream_t st);
 // workgroups of the T -> T2 halo copy of one box
 int64_t halo_copy_tiles(const Geom &g);
@@ -316,6 +318,10 @@ hipError_t launch_negt_repair(const Block &B, int *neg, hipStream_t st, int uvw 
 // block can take it, and the launch (writes u2, v2, w2 and T2)
 bool visc_fused_supported(const Block &B);
 hipError_t launch_visc_fused(const Block &B, int krk, int refresh, int *neg, hipStream_t st);
+// the same over the boxes of a batch (mixed precision; boxes whose handles
+// run it alone), and its workgroups for one box at segment length kseg
+hipError_t launch_visc_fused_batch(const BoxSet &S, int krk, int *neg, int ntiles, hipStream_t st);
+int64_t fused_batch_tiles(const Geom &g, int kseg);
 // workgroups of the show_flow_msg reduction (3 partial sums each)
 int64_t flow_msg_partials(const Geom &g);
 // largest CFL rate over the cells, as the bit pattern of a positive double,

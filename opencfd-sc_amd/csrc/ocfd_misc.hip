@@ -260,7 +260,8 @@ __global__ __launch_bounds__(NT) void k_negt_repair_batch(const BoxSet S, const 
   }
   const int t = blockIdx.x - nbox;
   const int j = S.entry(t);
-  halo_copy_tile(S.blk[S.ids[j]], t - S.off[j], 0);
+  const Block &B = S.blk[S.ids[j]];
+  halo_copy_tile(B, t - S.off[j], B.u2.p != nullptr ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

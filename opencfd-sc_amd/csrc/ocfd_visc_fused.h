@@ -50,6 +50,13 @@ inline int fused_kseg(const Geom &g) {
 inline unsigned fused_blocks(const Geom &g, int kseg) {
   return (unsigned)((g.nx / FTX) * (g.ny / FTY) * ((g.nz + kseg - 1) / kseg));
 }
+// planes per workgroup of a batch's fused march: the batch's segment length
+// (BATCH_KSEGS), at least 16 (each segment recomputes 2 x 3 planes of Ev3
+// around it), at most the box
+__host__ __device__ inline int fused_batch_kseg(int nz, int kseg) {
+  const int k = kseg < 16 ? 16 : kseg;
+  return k < nz ? k : nz;
+}
 
 template <typename RT>
 __device__ __forceinline__ void visc_fused_tile(const Block &B, const int krk, const int refresh, const int kseg,
@@ -326,6 +333,25 @@ template <typename RT>
 __global__ __launch_bounds__(NT) void k_visc_fused(const Block B, const int krk, const int refresh, const int kseg,
                                                    int *neg) {
   visc_fused_tile<RT>(B, krk, refresh, kseg, neg, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batches of periodic boxes (ocfd_batch): the tiles of the launch's boxes in
+// XCD order; box b marches segments of fused_batch_kseg planes and counts
+// its T <= 0 cells in neg + NEG_STRIDE * b.  Every box of the launch has
+// whole 32 x 8 tiles, CD6 and its second u, v, w buffers.
+template <typename RT>
+__global__ __launch_bounds__(NT) void k_visc_fused_batch(const BoxSet S, const int krk, int *neg) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = S.entry(t);
+  const int box = S.ids[j];
+  const Block B = S.blk[box];  // copied before any store, so its fields stay in scalar registers
+  visc_fused_tile<RT>(B, krk, 1, fused_batch_kseg(B.g.nz, B.kseg), neg + NEG_STRIDE * box, t - S.off[j]);
+}
+
+template <typename RT>
+hipError_t visc_fused_batch_as(const BoxSet &S, int krk, int *neg, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_visc_fused_batch<RT>, dim3((unsigned)ntiles), dim3(NT), 0, st, S, krk, neg);
+  return hipGetLastError();
 }
 
 template <typename RT>

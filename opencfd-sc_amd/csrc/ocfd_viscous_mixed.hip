@@ -18,6 +18,12 @@ hipError_t launch_visc_fused_mixed(const Block &B, int krk, int refresh, int *ne
   return visc_fused_as<float>(B, krk, refresh, neg, st);
 }
 
+hipError_t launch_visc_fused_batch(const BoxSet &S, int krk, int *neg, int ntiles, hipStream_t st) {
+  return visc_fused_batch_as<float>(S, krk, neg, ntiles, st);
+}
+
+int64_t fused_batch_tiles(const Geom &g, int kseg) { return fused_blocks(g, fused_batch_kseg(g.nz, kseg)); }
+
 hipError_t launch_visc_flux_batch_mixed(const BoxSet &S, bool cd8, int ntiles, hipStream_t st) {
   return visc_flux_batch_as<float>(S, cd8, ntiles, st);
 }

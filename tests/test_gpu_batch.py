@@ -38,6 +38,19 @@ QBOXES = [
 ]
 
 
+# mixed-precision boxes that step alone with the fused viscous stage update
+# (whole 32 x 8 tiles, CD6, at least 16 planes), beside boxes that do not:
+# nx = 48, CD8, inviscid.  Shapes are (nz, ny, nx).
+FBOXES = [
+    dict(shape=(24, 16, 32), seed=101, scheme="WENO5", split=1, char=0),
+    dict(shape=(16, 32, 64), seed=102, scheme="OMP6", split=1, char=0, noise=0.03),
+    dict(shape=(40, 8, 32), seed=103, scheme="WENO7", split=2, char=1),
+    dict(shape=(20, 24, 48), seed=104, scheme="WENO5", split=1, char=0),
+    dict(shape=(24, 16, 32), seed=105, scheme="UD7L", split=2, char=0, vis="CD8"),
+    dict(shape=(16, 16, 32), seed=106, scheme="WENO5", split=1, char=0, inviscid=True),
+]
+
+
 def make_cases(precision, boxes=BOXES):
     out = []
     for kw in boxes:
@@ -103,6 +116,33 @@ def test_batch_march_shapes(precision):
             batch.set_sweep_tiles(gi, [3 + gi % 2, 3 + gi % 2, 4 - gi % 2])
         assert batch.step(3) == 0
         check_equal(batch, ref3, "3 batched steps on the marches")
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_batch_fused_viscous_update(graph, monkeypatch):
+    """Mixed-precision boxes whose handles run k_visc_fused alone run it in
+    the batch too (their u, v, w sets swap with T every stage): every box
+    equals the box stepped alone, after an odd and an even number of steps
+    and after the autotuner."""
+    if graph:
+        monkeypatch.delenv("OCFD_NO_GRAPH", raising=False)
+    else:
+        monkeypatch.setenv("OCFD_NO_GRAPH", "1")
+    cs = make_cases(PREC_MIXED, FBOXES)
+    with native.setup_block(native.Block(cs[0].cfg), cs[0], use_uniform=True) as b:
+        b.profile_enable(True)
+        b.step(1)
+        assert "stage_update_fused" in b.profile_read()
+    ref3 = single_runs(cs, 3)
+    ref5 = single_runs(cs, 5)
+    with native.Batch([c.cfg for c in cs]) as batch:
+        for i, c in enumerate(cs):
+            native.setup_block(batch.box(i), c, use_uniform=True)
+        assert batch.step(3) == 0
+        check_equal(batch, ref3, "3 batched steps")
+        batch.autotune(reps=2)
+        assert batch.step(2) == 0
+        check_equal(batch, ref5, "2 more steps after autotune")
 
 
 def test_batch_rejects_unsupported_boxes():
