@@ -425,8 +425,9 @@ def run_batch(args):
                    "initial": "Taylor-Green or seeded isotropic per box; Ma U[0.1,0.5], Re U[400,3200]",
                    "scheme": "WENO5 or OMP6 per box, Steger-Warming, CD6, RK3 CFL 0.5",
                    "precision": "FP64" if prec == PREC_FP64 else "mixed",
-                   "execution": "one launch per kernel phase over all boxes, graph-captured steps (the "
-                                "persistent work-queue launch is built but switched off: its kernel did not "
+                   "execution": "one launch per kernel phase over all boxes, graph-captured steps; in mixed "
+                                "precision the boxes with whole 32x8 tiles take the fused viscous stage update "
+                                "(the persistent work-queue launch is built but switched off: its kernel did not "
                                 "finish on the GPU)",
                    "autotuned": tiles},
         "one_at_a_time": {"value": round(single, 3), "ms_per_step": round(el_s / args.steps * 1e3, 4),
