@@ -308,10 +308,23 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   // item's index arithmetic, slot and closure tests are done once per face
   constexpr int NIT = CHAR ? NF * TW : 5 * NF * TW;
   constexpr int NI2 = (NIT + NTHREADS - 1) / NTHREADS;
+  // The hybrid scheme's characteristic faces are linear, listed or
+  // closures, none projected here: (TL + 1) x TW faces leave TW faces past
+  // the last whole round of lanes, which go one component per lane (a
+  // wavefront's round for a face with all five would cost as much as the
+  // 64 faces of a full one)
+  constexpr int FULL = (NIT / NTHREADS) * NTHREADS;
+  constexpr bool SPLIT_TAIL = CHAR && HYB && NIT % NTHREADS != 0 && 5 * (NIT - FULL) <= NTHREADS;
   constexpr int NV = CHAR ? 5 : 2;  // face-array values per item
   R hv[ALIAS ? NI2 : 1][NV];
   // item r -> its face-array entries; false for items outside the tile
   auto decode = [&](int r, int &m, int &f, int &wl) {
+    if (SPLIT_TAIL && r == NI2 - 1) {
+      if (tid >= 5 * (NIT - FULL)) return false;
+      m = tid % 5;
+      item_ch(FULL + tid / 5, f, wl);
+      return s0 - 1 + f <= n && w0 + wl <= nw;
+    }
     const int it = tid + r * NTHREADS;
     if (it >= NIT) return false;
     if (CHAR) { m = 0; item_ch(it, f, wl); } else item_nc(it, m, f, wl);
@@ -351,6 +364,11 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
         } else {
           face_ch(f, wl, s, 0, v);
         }
+      } else if (SPLIT_TAIL && r == NI2 - 1) {
+        // hybrid, characteristic, past the whole rounds: component m
+        if ((closure_lo && s == 0) || (closure_hi && s == n)) v[0] = closure_comp(f, wl, s, m);
+        else if (slot_of(f, wl) >= 0) v[0] = R(B.wflux[AXIS][(int64_t)slot_of(f, wl) * NH + m]);
+        else v[0] = linear_comp(f, wl, s, m);
       } else {
         // hybrid, characteristic: the closure, listed and linear faces
         if ((closure_lo && s == 0) || (closure_hi && s == n)) {
@@ -374,7 +392,8 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     for (int r = 0; r < NI2; r++) {
       int m, f, wl;
       if (!decode(r, m, f, wl)) continue;
-      put(m, f, wl, hv[r]);
+      if (SPLIT_TAIL && r == NI2 - 1) sH[lidx<AXIS, NF, TW>(m, f, wl)] = hv[r][0];
+      else put(m, f, wl, hv[r]);
     }
   } else {
     // separate face array: each item's fluxes are stored as computed
@@ -405,8 +424,9 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
       }
     } else {
       // hybrid, characteristic: the linear and closure faces, all five
-      // components per item
-      for (int it = tid; it < NF * TW; it += NTHREADS) {
+      // components per item, those past the whole rounds one component per
+      // lane
+      for (int it = tid; it < (SPLIT_TAIL ? FULL : NF * TW); it += NTHREADS) {
         int f, wl;
         item_ch(it, f, wl);
         const int s = s0 - 1 + f;
@@ -422,6 +442,21 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
           continue;  // written in phase 1
         }
         put(0, f, wl, v);
+      }
+      if constexpr (SPLIT_TAIL) {
+        if (tid < 5 * (NIT - FULL)) {
+          const int m = tid % 5;
+          int f, wl;
+          item_ch(FULL + tid / 5, f, wl);
+          const int s = s0 - 1 + f;
+          if (s <= n && w0 + wl <= nw) {
+            if ((closure_lo && s == 0) || (closure_hi && s == n))
+              sH[lidx<AXIS, NF, TW>(m, f, wl)] = closure_comp(f, wl, s, m);
+            else if (slot_of(f, wl) < 0)
+              sH[lidx<AXIS, NF, TW>(m, f, wl)] = linear_comp(f, wl, s, m);
+            // listed faces: written in phase 1
+          }
+        }
       }
     }
   }
