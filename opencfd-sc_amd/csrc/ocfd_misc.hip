@@ -128,21 +128,28 @@ __global__ __launch_bounds__(NT) void k_sensor2b(const Block B, const double *su
 __global__ __launch_bounds__(NT) void k_hyb_counts(const Block B, unsigned long long *counts) {
   const Geom &g = B.g;
   const int64_t nf = (int64_t)(g.nx + 1) * g.ny * g.nz;
-  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  int sc = 0;
-  if (t < nf) {
+  // a bounded grid strides over the faces and adds one total per scheme and
+  // workgroup: one global atomic per workgroup of 256 faces serialised
+  // half a million atomics on three addresses at 512^3
+  unsigned c[3] = {0, 0, 0};
+  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < nf; t += (int64_t)gridDim.x * NT) {
     const int i = (int)(t % (g.nx + 1));
     const int64_t r = t / (g.nx + 1);
     const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
     const double r0 = dmax2(B.rhyb[g.hi(i, j, k)], B.rhyb[g.hi(i + 1, j, k)]);
-    sc = hybrid_index(r0, B.S.hyb2, B.S.hyb3);
+    const int sc = hybrid_index(r0, B.S.hyb2, B.S.hyb3);
+    c[0] += sc == 1;
+    c[1] += sc == 2;
+    c[2] += sc == 3;
   }
-  __shared__ unsigned cnt[3];
+  __shared__ unsigned long long cnt[3];
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
   __syncthreads();
-  if (sc) atomicAdd(&cnt[sc - 1], 1u);
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+    if (c[q]) atomicAdd(&cnt[q], (unsigned long long)c[q]);
   __syncthreads();
-  if (threadIdx.x < 3 && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+  if (threadIdx.x < 3 && cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], cnt[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -892,7 +899,8 @@ hipError_t launch_sensor2(const Block &B, double *work, hipStream_t st) {
 
 hipError_t launch_hybrid_counts(const Block &B, unsigned long long *counts, hipStream_t st) {
   const int64_t nf = (int64_t)(B.g.nx + 1) * B.g.ny * B.g.nz;
-  hipLaunchKernelGGL(k_hyb_counts, dim3(nb(nf)), dim3(NT), 0, st, B, counts);
+  const unsigned nblk = nb(nf);
+  hipLaunchKernelGGL(k_hyb_counts, dim3(nblk < 2048 ? nblk : 2048), dim3(NT), 0, st, B, counts);
   return hipGetLastError();
 }
 
