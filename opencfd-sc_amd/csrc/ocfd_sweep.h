@@ -308,13 +308,31 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
   // item's index arithmetic, slot and closure tests are done once per face
   constexpr int NIT = CHAR ? NF * TW : 5 * NF * TW;
   constexpr int NI2 = (NIT + NTHREADS - 1) / NTHREADS;
-  // The hybrid scheme's characteristic faces are linear, listed or
-  // closures, none projected here: (TL + 1) x TW faces leave TW faces past
-  // the last whole round of lanes, which go one component per lane (a
-  // wavefront's round for a face with all five would cost as much as the
-  // 64 faces of a full one)
+  // On the characteristic path (TL + 1) x TW faces leave TW faces past the
+  // last whole round of lanes; a wavefront's round for a face with all five
+  // components would cost as much as the 64 faces of a full one, so these
+  // go five lanes to a face.  The hybrid scheme's faces (linear, listed or
+  // closures, none projected here) take one component per lane; a
+  // projected face takes one characteristic row per lane (its basis formed
+  // by each), the five rows meet in LDS (sTail), and each lane projects one
+  // component back.
   constexpr int FULL = (NIT / NTHREADS) * NTHREADS;
-  constexpr bool SPLIT_TAIL = CHAR && HYB && NIT % NTHREADS != 0 && 5 * (NIT - FULL) <= NTHREADS;
+  constexpr bool SPLIT_TAIL = CHAR && NIT % NTHREADS != 0 && 5 * (NIT - FULL) <= NTHREADS;
+  R *const sTail = reinterpret_cast<R *>(sS);     // projecting tiles: tail rows [5 * TW]
+  // one tail item: face FULL + tid / 5, component / row tid % 5; stage 1 of
+  // a projected face leaves its row in sTail, stage 2 (after a barrier)
+  // returns the component
+  auto tail_row = [&](CharBasisT<R> &Bs, int f, int wl, int s, int row) {
+    basis_of(f, wl, Bs);
+    sTail[tid] = char_row(Bs, f, wl, s, 0, row);
+  };
+  auto tail_back = [&](const CharBasisT<R> &Bs, int row) {
+    R h0[5], hh[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) h0[q] = sTail[(tid / 5) * 5 + q];
+    char_back(Bs, h0, hh);
+    return hh[row];
+  };
   constexpr int NV = CHAR ? 5 : 2;  // face-array values per item
   R hv[ALIAS ? NI2 : 1][NV];
   // item r -> its face-array entries; false for items outside the tile
@@ -345,7 +363,20 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     // results held in registers until every stencil has been read
 #pragma unroll
     for (int r = 0; r < NI2; r++) {
-      int m, f, wl;
+      int m = 0, f = 0, wl = 0;
+      if constexpr (SPLIT_TAIL && !HYB) {
+        if (r == NI2 - 1) {
+          // projected tail faces: a row per lane, a barrier, a component back
+          const bool act = decode(r, m, f, wl);
+          const int s = s0 - 1 + f;
+          const bool clo = (closure_lo && s == 0) || (closure_hi && s == n);
+          CharBasisT<R> Bs;
+          if (act && !clo) tail_row(Bs, f, wl, s, m);
+          __syncthreads();
+          if (act) hv[r][0] = clo ? closure_comp(f, wl, s, m) : tail_back(Bs, m);
+          continue;
+        }
+      }
       if (!decode(r, m, f, wl)) continue;
       const int s = s0 - 1 + f;
       R v[NV];
@@ -409,7 +440,7 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
         put(m, f, wl, v);
       }
     } else if constexpr (!HYB) {
-      for (int it = tid; it < NF * TW; it += NTHREADS) {
+      for (int it = tid; it < (SPLIT_TAIL ? FULL : NF * TW); it += NTHREADS) {
         int f, wl;
         item_ch(it, f, wl);
         const int s = s0 - 1 + f;
@@ -421,6 +452,16 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
           face_ch(f, wl, s, 0, v);
         }
         put(0, f, wl, v);
+      }
+      if constexpr (SPLIT_TAIL) {
+        int m = 0, f = 0, wl = 0;
+        const bool act = decode(NI2 - 1, m, f, wl);
+        const int s = s0 - 1 + f;
+        const bool clo = (closure_lo && s == 0) || (closure_hi && s == n);
+        CharBasisT<R> Bs;
+        if (act && !clo) tail_row(Bs, f, wl, s, m);
+        __syncthreads();
+        if (act) sH[lidx<AXIS, NF, TW>(m, f, wl)] = clo ? closure_comp(f, wl, s, m) : tail_back(Bs, m);
       }
     } else {
       // hybrid, characteristic: the linear and closure faces, all five
@@ -520,7 +561,7 @@ __global__ __launch_bounds__(NTHREADS) void k_inviscid_batch(const BoxSet S, con
 template <int TL, int TW, bool CHAR, bool HYB, typename R>
 constexpr size_t sweep_lds(int nq) {
   return sizeof(R) * (10 * (TL + 8) * TW + (CHAR && !HYB ? nq * (TL + 8) * TW : 0) +
-                      (sizeof(R) == 8 ? 0 : (CHAR ? 5 : 10) * (TL + 1) * TW)) +
+                      (sizeof(R) == 8 ? 0 : (CHAR ? 5 : 10) * (TL + 1) * TW) + (CHAR && !HYB ? 5 * TW : 0)) +
          (HYB ? sizeof(int) * (TL + 1) * TW : 0);
 }
 
