@@ -289,6 +289,9 @@ __device__ __forceinline__ void sweep_tile(const Block &B, int b, const int mode
     CharBasisT<R> Bs;
     basis_of(f, wl, Bs);
     R h0[5];
+    // one row at a time: unrolled, the five rows' stencils stayed live
+    // together (FP32 tiles at 256 VGPRs plus AGPRs, one wave per SIMD)
+#pragma unroll 1
     for (int r = 0; r < 5; r++) h0[r] = char_row(Bs, f, wl, s, scm, r);
     char_back(Bs, h0, hh);
   };
